@@ -1,0 +1,273 @@
+"""w3914777_amd — MI355X adaptive mixed-precision phase-field stencil engine.
+
+Thin Python view of the C ABI in include/amps.h (libamps.so, built for
+gfx950 by w3914777_amd/build.py).  The engine itself is C++ host code plus
+hand-written HIP kernels; this module only marshals configurations and FP64
+host arrays.  There is no fallback: if libamps.so is missing or the HIP
+runtime cannot be used, every call raises.
+
+Typical use mirrors one reference driver run (e.g. snowflake/end2end_mix.cu):
+
+    cfg = Config.default(SNOWFLAKE); cfg.mode = ADAPTIVE
+    with Context(cfg) as ctx:
+        ctx.init(); ctx.step(500); ms = ctx.sync()
+        phi = ctx.get_field(0)
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libamps.so")
+
+SNOWFLAKE, GRAINGROWTH, SINTERING = 1, 2, 3
+PURE_FP64, PURE_HALF2, ADAPTIVE, STATIC_MAP = 0, 1, 2, 3
+MON_NONE, MON_SPATIAL, MON_TEMPORAL, MON_GRADIENT = 0, 1, 2, 3
+STENCIL_REF, STENCIL_LAP5 = 0, 1
+BC_PERIODIC, BC_NEUMANN = 0, 1
+INIT_REF, INIT_CONFIG = 0, 1
+NPARAMS = 32
+
+SF = dict(DX=0, DY=1, DT=2, TAU=3, EPSB=4, KAPPA=5, DELTA=6, ANISO=7, ALPHA=8, GAMMA=9,
+          TEQ=10, THETA0=11, PI=12, NOISE=13, R2=14, CX=15, CY=16)
+GG = dict(DX=0, DY=1, DT=2, L=3, KAPPA=4, ALPHA=5, BETA=6, GAMMA=7, R=8, CX=9, CY=10, AMP=11)
+SN = dict(DX=0, DY=1, DT=2, A=3, B=4, KC=5, KE=6, L=7, DVOL=8, DVAP=9, DSURF=10, DGB=11,
+          R1=12, R2=13, X1=14, Y1=15, X2=16, Y2=17, WIDTH=18, NOISE=19)
+GRAD_SF = dict(DELTA=1, KAPPA=2, EPSB=3, TAU=4)
+GRAD_SN = dict(DSURF=101, DGB=102, KC=103, KE=104, L=105, DVOL=106)
+
+STATUS = {0: "ok", -1: "invalid argument", -2: "out of memory", -3: "HIP runtime error",
+          -4: "parameter file parse error", -5: "non-finite value in a field", -6: "call not valid in this state",
+          -7: "unsupported configuration", -8: "I/O error"}
+
+
+class AmpsError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"amps status {status} ({STATUS.get(status, '?')}): {message}")
+        self.status = status
+
+
+class Config(ctypes.Structure):
+    """Mirror of amps_config (include/amps.h)."""
+    _fields_ = [("model", ctypes.c_int32), ("mode", ctypes.c_int32), ("monitor", ctypes.c_int32),
+                ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("q", ctypes.c_int32), ("tile", ctypes.c_int32),
+                ("monitor_every", ctypes.c_int32), ("monitor_field", ctypes.c_int32),
+                ("stencil", ctypes.c_int32), ("bc", ctypes.c_int32), ("init", ctypes.c_int32),
+                ("threshold", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("params", ctypes.c_double * NPARAMS)]
+
+    @classmethod
+    def default(cls, model):
+        c = cls()
+        _check(lib().amps_default_config(model, ctypes.byref(c)))
+        return c
+
+    @classmethod
+    def from_paras(cls, model, path, base=None):
+        """Reference-syntax parameter file -> (Config, timesteps or None)."""
+        c = cls.default(model) if base is None else base.copy()
+        ts = ctypes.c_int(-1)
+        _check(lib().amps_load_paras(os.fsencode(path), ctypes.byref(c), ctypes.byref(ts)))
+        return c, (ts.value if ts.value >= 0 else None)
+
+    def copy(self):
+        c = Config()
+        ctypes.memmove(ctypes.byref(c), ctypes.byref(self), ctypes.sizeof(Config))
+        return c
+
+    def set_params(self, values):
+        for k, v in values.items():
+            self.params[k] = v
+        return self
+
+    def nfields(self):
+        return {SNOWFLAKE: 2, GRAINGROWTH: self.q, SINTERING: 3}[self.model]
+
+
+class Info(ctypes.Structure):
+    _fields_ = [("steps_done", ctypes.c_int64), ("monitor_passes", ctypes.c_int64), ("ntiles", ctypes.c_int32),
+                ("type_groups", ctypes.c_int32), ("nfields", ctypes.c_int32), ("ninstances", ctypes.c_int32),
+                ("hi_frac_last", ctypes.c_double), ("hi_frac_mean", ctypes.c_double), ("cells", ctypes.c_double)]
+
+
+_lib = None
+
+EXPORTS = ["amps_default_config", "amps_load_paras", "amps_create", "amps_create_batch", "amps_destroy",
+           "amps_init", "amps_step", "amps_sync", "amps_monitor_now", "amps_get_field", "amps_set_field",
+           "amps_get_tile_types", "amps_set_tile_types", "amps_get_monitor_values", "amps_get_info",
+           "amps_set_kernel_timing", "amps_get_kernel_time",
+           "amps_tape_begin", "amps_adjoint", "amps_status_string", "amps_last_error"]
+
+
+def lib():
+    """Load libamps.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise AmpsError(-7, f"{LIB_PATH} is missing: build it with __graft_entry__.build() or "
+                            "python w3914777_amd/build.py")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    D = ctypes.POINTER(ctypes.c_double)
+    I32P = ctypes.POINTER(ctypes.c_int32)
+    L.amps_default_config.argtypes = [ctypes.c_int, ctypes.POINTER(Config)]
+    L.amps_load_paras.argtypes = [ctypes.c_char_p, ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_int)]
+    L.amps_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(P)]
+    L.amps_create_batch.argtypes = [ctypes.POINTER(Config), ctypes.c_int, ctypes.POINTER(P)]
+    L.amps_destroy.argtypes = [P]
+    L.amps_destroy.restype = None
+    L.amps_init.argtypes = [P]
+    L.amps_step.argtypes = [P, ctypes.c_int]
+    L.amps_sync.argtypes = [P, D]
+    L.amps_monitor_now.argtypes = [P]
+    L.amps_get_field.argtypes = [P, ctypes.c_int, ctypes.c_int, D, ctypes.c_size_t]
+    L.amps_set_field.argtypes = [P, ctypes.c_int, ctypes.c_int, D, ctypes.c_size_t]
+    L.amps_get_tile_types.argtypes = [P, ctypes.c_int, I32P, ctypes.c_size_t]
+    L.amps_set_tile_types.argtypes = [P, ctypes.c_int, I32P, ctypes.c_size_t]
+    L.amps_get_monitor_values.argtypes = [P, ctypes.c_int, D, ctypes.c_size_t]
+    L.amps_get_info.argtypes = [P, ctypes.POINTER(Info)]
+    L.amps_set_kernel_timing.argtypes = [P, ctypes.c_int]
+    L.amps_get_kernel_time.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]
+    L.amps_tape_begin.argtypes = [P, ctypes.c_int]
+    L.amps_adjoint.argtypes = [P, D, I32P, ctypes.c_int, D, D]
+    L.amps_status_string.argtypes = [ctypes.c_int]
+    L.amps_status_string.restype = ctypes.c_char_p
+    L.amps_last_error.argtypes = []
+    L.amps_last_error.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def _check(status):
+    if status != 0:
+        raise AmpsError(status, lib().amps_last_error().decode(errors="replace"))
+    return status
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _iptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+class Context:
+    """One amps_ctx: a grid (or a batched ensemble of grids) on the current GPU."""
+
+    def __init__(self, cfgs):
+        if isinstance(cfgs, Config):
+            cfgs = [cfgs]
+        self.cfgs = [c.copy() for c in cfgs]
+        arr = (Config * len(cfgs))(*self.cfgs)
+        h = ctypes.c_void_p()
+        _check(lib().amps_create_batch(arr, len(cfgs), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().amps_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init(self):
+        _check(lib().amps_init(self._h))
+
+    def step(self, n):
+        _check(lib().amps_step(self._h, int(n)))
+
+    def sync(self):
+        ms = ctypes.c_double(0)
+        _check(lib().amps_sync(self._h, ctypes.byref(ms)))
+        return ms.value
+
+    def monitor_now(self):
+        _check(lib().amps_monitor_now(self._h))
+
+    def shape(self, inst=0):
+        c = self.cfgs[inst]
+        return (c.ny, c.nx)
+
+    def get_field(self, field, inst=0):
+        ny, nx = self.shape(inst)
+        out = np.empty((ny, nx), np.float64)
+        _check(lib().amps_get_field(self._h, inst, field, _dptr(out), out.size))
+        return out
+
+    def set_field(self, field, values, inst=0):
+        ny, nx = self.shape(inst)
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        if a.shape != (ny, nx):
+            raise ValueError(f"expected shape {(ny, nx)}, got {a.shape}")
+        _check(lib().amps_set_field(self._h, inst, field, _dptr(a), a.size))
+
+    def _ntypes(self, inst):
+        c = self.cfgs[inst]
+        groups = c.q if c.model == GRAINGROWTH else 1
+        return groups, (c.ny // c.tile), (c.nx // c.tile)
+
+    def get_tile_types(self, inst=0):
+        g, ty, tx = self._ntypes(inst)
+        out = np.empty((g, ty, tx), np.int32)
+        _check(lib().amps_get_tile_types(self._h, inst, _iptr(out), out.size))
+        return out if g > 1 else out[0]
+
+    def set_tile_types(self, types, inst=0):
+        g, ty, tx = self._ntypes(inst)
+        a = np.ascontiguousarray(np.broadcast_to(types, (g, ty, tx)), dtype=np.int32)
+        _check(lib().amps_set_tile_types(self._h, inst, _iptr(a), a.size))
+
+    def get_monitor_values(self, inst=0):
+        g, ty, tx = self._ntypes(inst)
+        out = np.empty((g, ty, tx), np.float64)
+        _check(lib().amps_get_monitor_values(self._h, inst, _dptr(out), out.size))
+        return out if g > 1 else out[0]
+
+    def info(self):
+        i = Info()
+        _check(lib().amps_get_info(self._h, ctypes.byref(i)))
+        return i
+
+    def set_kernel_timing(self, enable=True):
+        _check(lib().amps_set_kernel_timing(self._h, 1 if enable else 0))
+
+    def kernel_time(self):
+        """(launches, total_ms) of the sweep kernel since timing was enabled."""
+        n = ctypes.c_int64(0)
+        ms = ctypes.c_double(0)
+        _check(lib().amps_get_kernel_time(self._h, ctypes.byref(n), ctypes.byref(ms)))
+        return n.value, ms.value
+
+    def tape_begin(self, ckpt_every):
+        _check(lib().amps_tape_begin(self._h, int(ckpt_every)))
+
+    def adjoint(self, dJ_du, param_ids, want_du0=False):
+        """dJ_du: list over instances of [nfields, ny, nx] cotangents.
+        Returns (dJ_dp [ninst, np], dJ_du0 list or None)."""
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float64).ravel() for x in dJ_du]))
+        pid = np.ascontiguousarray(param_ids, dtype=np.int32)
+        out = np.zeros((len(self.cfgs), len(pid)), np.float64)
+        du0 = np.zeros_like(flat) if want_du0 else None
+        _check(lib().amps_adjoint(self._h, _dptr(flat), _iptr(pid), len(pid), _dptr(out),
+                                  _dptr(du0) if want_du0 else None))
+        if not want_du0:
+            return out, None
+        res, o = [], 0
+        for c in self.cfgs:
+            n = c.nfields() * c.nx * c.ny
+            res.append(du0[o:o + n].reshape(c.nfields(), c.ny, c.nx))
+            o += n
+        return out, res

@@ -1,0 +1,815 @@
+// api.cpp — host runtime of the C ABI declared in include/amps.h.
+//
+// A context holds every instance of a run (one grid, or an ensemble of
+// uneven grids of one model), all device buffers, one HIP stream and the
+// captured step graphs.  amps_step enqueues work without host
+// synchronisation: the monitor cadence, tile classification, conversion and
+// the explicit-Euler sweeps are all device work, replayed from hipGraphs in
+// blocks of one monitor period.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+using namespace amps;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                        \
+    do {                                                                                                    \
+        hipError_t e_ = (expr);                                                                             \
+        if (e_ != hipSuccess)                                                                               \
+            return fail(AMPS_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));                    \
+    } while (0)
+
+#define RCCHK(expr)                                                                                         \
+    do {                                                                                                    \
+        int r_ = (expr);                                                                                    \
+        if (r_ != AMPS_OK) return r_ == AMPS_E_HIP ? fail(r_, std::string(#expr) + ": " +                  \
+                                                        hipGetErrorString(hipGetLastError()))               \
+                                                  : r_;                                                     \
+    } while (0)
+
+int nfields_of(const amps_config& c) {
+    if (c.model == AMPS_SNOWFLAKE) return 2;
+    if (c.model == AMPS_GRAINGROWTH) return c.q;
+    return 3;
+}
+
+}  // namespace
+
+namespace amps {
+void set_error_text(const char* msg) { g_last_error = msg ? msg : ""; }
+}  // namespace amps
+
+struct amps_ctx {
+    int model = 0, mode = 0, monitor = 0, tile = 16, every = 10, mfield = 0;
+    int nfields = 0, ngroups = 1, q = 0, ninst = 0;
+    double thr_eff = 0;
+    std::vector<amps_config> cfg;
+    std::vector<InstDesc> desc;
+    std::vector<int> tile_inst;
+    long long total_cells = 0;
+    int total_tiles = 0;
+    long long max_inst_cells = 0;
+    GroupMap gm{};
+
+    InstDesc* d_desc = nullptr;
+    int* d_tile_inst = nullptr;
+    Buf buf[2]{};
+    int cur = 0;
+    int* d_type = nullptr;
+    int* d_type_old = nullptr;
+    double* d_md = nullptr;
+    double* d_last = nullptr;
+    unsigned long long* d_stats = nullptr;  // [0] sum of FP64 tiles over passes, [1] last pass
+    long long* d_step = nullptr;
+    double* d_aux[2] = {nullptr, nullptr};
+    double* d_scratch = nullptr;           // max instance cells (get/set)
+    int* d_wl = nullptr;                   // [2][total_tiles]: FP64 and low tile work-lists
+    int* d_wl_count = nullptr;             // [2]
+    int grid_hi = 0, grid_lo = 0;          // persistent grid sizes of the snowflake sweeps
+
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timing_open = false;
+
+    long long steps_done = 0;
+    long long monitor_passes = 0;
+    int pending_adv = 0;
+    bool initialised = false;
+
+    // graphs: one per starting buffer parity
+    bool use_graph = true;
+    int graph_period = 0;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+
+    // adjoint tape
+    int tape_every = 0;
+    bool taping = false;
+    long long tape_step0 = 0;
+    std::vector<double*> tape;              // device checkpoints (all fields)
+    std::vector<long long> tape_steps;
+
+    // sweep-kernel timing (events around each step-kernel launch)
+    bool ktiming = false;
+    std::vector<hipEvent_t> kev;  // pairs
+    int kev_used = 0;
+    long long klaunches = 0;
+    double kms = 0.0;
+
+    std::vector<void*> allocs;
+};
+
+int tape_push(amps_ctx* c);
+
+namespace {
+
+int dmalloc(amps_ctx* c, void** p, size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return fail(AMPS_E_NOMEM, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    c->allocs.push_back(*p);
+    HIPCHK(hipMemset(*p, 0, bytes));
+    return AMPS_OK;
+}
+
+bool finite_params(const amps_config& c) {
+    for (int i = 0; i < AMPS_NPARAMS; ++i)
+        if (!std::isfinite(c.params[i])) return false;
+    return std::isfinite(c.threshold);
+}
+
+int validate(const amps_config& c) {
+    if (c.model < 1 || c.model > 3) return fail(AMPS_E_ARG, "model must be 1 (snowflake), 2 (grain growth) or 3 (sintering)");
+    if (c.mode < 0 || c.mode > 3) return fail(AMPS_E_ARG, "mode out of range");
+    if (c.tile != 16 && c.tile != 32) return fail(AMPS_E_UNSUPPORTED, "tile must be 16 or 32");
+    if (c.nx < c.tile || c.ny < c.tile || c.nx % c.tile || c.ny % c.tile)
+        return fail(AMPS_E_ARG, "nx and ny must be positive multiples of tile");
+    if ((long long)c.nx * c.ny > (1ll << 31)) return fail(AMPS_E_ARG, "grid too large (> 2^31 cells)");
+    if (c.stencil != AMPS_STENCIL_REF && c.stencil != AMPS_STENCIL_LAP5) return fail(AMPS_E_ARG, "stencil out of range");
+    if (c.bc != AMPS_BC_PERIODIC && c.bc != AMPS_BC_NEUMANN) return fail(AMPS_E_ARG, "bc out of range");
+    if (c.init != AMPS_INIT_REF && c.init != AMPS_INIT_CONFIG) return fail(AMPS_E_ARG, "init out of range");
+    if (!finite_params(c)) return fail(AMPS_E_ARG, "non-finite parameter");
+    if (c.mode == AMPS_ADAPTIVE) {
+        if (c.monitor < 1 || c.monitor > 3) return fail(AMPS_E_ARG, "ADAPTIVE needs monitor 1, 2 or 3");
+        if (c.monitor_every < 1) return fail(AMPS_E_ARG, "monitor_every must be >= 1");
+    }
+    const double* p = c.params;
+    if (p[0] <= 0 || p[1] <= 0 || p[2] <= 0) return fail(AMPS_E_ARG, "dx, dy and dt must be positive");
+    if (c.model == AMPS_SNOWFLAKE) {
+        double n = p[AMPS_SF_ANISO];
+        if (n < 1 || n > 12 || n != std::floor(n)) return fail(AMPS_E_UNSUPPORTED, "snowflake aniso must be an integer in [1, 12]");
+        if (p[AMPS_SF_TAU] <= 0) return fail(AMPS_E_ARG, "tau must be positive");
+        // explicit Euler on the T diffusion: spectral radius of the cross-2
+        // operator is 12.5/(dx dy) (5-point: 8/(dx dy)); require dt*rho < 2
+        double rho = (c.stencil == AMPS_STENCIL_REF ? 12.5 : 8.0) / (p[AMPS_SF_DX] * p[AMPS_SF_DY]);
+        if (p[AMPS_SF_DT] * rho >= 2.0) return fail(AMPS_E_ARG, "dt violates the explicit-Euler bound of the T diffusion");
+        if (c.monitor_field != 0 && c.mode == AMPS_ADAPTIVE && (c.monitor_field < 0 || c.monitor_field > 1))
+            return fail(AMPS_E_ARG, "snowflake monitor_field must be 0 (phi) or 1 (T)");
+    } else if (c.model == AMPS_GRAINGROWTH) {
+        if (c.q < 2 || c.q > 38) return fail(AMPS_E_ARG, "grain growth q must be in [2, 38]");
+        double rho = (c.stencil == AMPS_STENCIL_REF ? 12.0 : 8.0) / (p[AMPS_GG_DX] * p[AMPS_GG_DY]);
+        if (p[AMPS_GG_DT] * p[AMPS_GG_L] * p[AMPS_GG_KAPPA] * rho >= 2.0)
+            return fail(AMPS_E_ARG, "dt*L*kappa violates the explicit-Euler bound");
+        if (c.mode == AMPS_ADAPTIVE && (c.monitor_field < -1 || c.monitor_field >= c.q))
+            return fail(AMPS_E_ARG, "grain growth monitor_field must be -1 or a field index");
+    } else {
+        if (c.mode == AMPS_ADAPTIVE && (c.monitor_field < 0 || c.monitor_field > 2))
+            return fail(AMPS_E_ARG, "sintering monitor_field must be 0, 1 or 2");
+    }
+    return AMPS_OK;
+}
+
+void derive(InstDesc& d, const amps_config& c) {
+    const double* p = c.params;
+    for (int i = 0; i < AMPS_NPARAMS; ++i) d.p[i] = p[i];
+    for (int i = 0; i < 24; ++i) d.k[i] = 0;
+    if (c.model == AMPS_SNOWFLAKE) {
+        const double n = p[AMPS_SF_ANISO];
+        d.k[SFK_INV_DXDY] = 1.0 / (p[AMPS_SF_DX] * p[AMPS_SF_DY]);
+        d.k[SFK_INV_2DX] = 1.0 / (2.0 * p[AMPS_SF_DX]);
+        d.k[SFK_INV_2DY] = 1.0 / (2.0 * p[AMPS_SF_DY]);
+        d.k[SFK_DT_OVER_TAU] = p[AMPS_SF_DT] / p[AMPS_SF_TAU];
+        d.k[SFK_DT] = p[AMPS_SF_DT];
+        d.k[SFK_KAPPA] = p[AMPS_SF_KAPPA];
+        d.k[SFK_ALPHA_OVER_PI] = p[AMPS_SF_ALPHA] / p[AMPS_SF_PI];
+        d.k[SFK_GAMMA] = p[AMPS_SF_GAMMA];
+        d.k[SFK_TEQ] = p[AMPS_SF_TEQ];
+        d.k[SFK_EPSB] = p[AMPS_SF_EPSB];
+        d.k[SFK_EPSB_DELTA] = p[AMPS_SF_EPSB] * p[AMPS_SF_DELTA];
+        d.k[SFK_MEPSB_ANISO_DELTA] = -p[AMPS_SF_EPSB] * n * p[AMPS_SF_DELTA];
+        d.k[SFK_COS_N_THETA0] = std::cos(n * p[AMPS_SF_THETA0]);
+        d.k[SFK_SIN_N_THETA0] = std::sin(n * p[AMPS_SF_THETA0]);
+        d.k[SFK_NOISE] = p[AMPS_SF_NOISE];
+        d.k[SFK_ANISO_N] = n;
+    } else if (c.model == AMPS_GRAINGROWTH) {
+        d.k[GGK_INV_DXDY] = 1.0 / (p[AMPS_GG_DX] * p[AMPS_GG_DY]);
+        d.k[GGK_DT_L] = p[AMPS_GG_DT] * p[AMPS_GG_L];
+        d.k[GGK_KAPPA] = p[AMPS_GG_KAPPA];
+        d.k[GGK_ALPHA] = p[AMPS_GG_ALPHA];
+        d.k[GGK_BETA] = p[AMPS_GG_BETA];
+        d.k[GGK_2GAMMA] = 2.0 * p[AMPS_GG_GAMMA];
+    } else {
+        d.k[SNK_INV_DXDY] = 1.0 / (p[AMPS_SN_DX] * p[AMPS_SN_DY]);
+        d.k[SNK_DT] = p[AMPS_SN_DT];
+        d.k[SNK_A2] = 2.0 * p[AMPS_SN_A];
+        d.k[SNK_B] = p[AMPS_SN_B];
+        d.k[SNK_HALF_KC] = 0.5 * p[AMPS_SN_KC];
+        d.k[SNK_DVOL] = p[AMPS_SN_DVOL];
+        d.k[SNK_DVAP] = p[AMPS_SN_DVAP];
+        d.k[SNK_DSURF] = p[AMPS_SN_DSURF];
+        d.k[SNK_DGB] = p[AMPS_SN_DGB];
+        d.k[SNK_DT_L] = p[AMPS_SN_DT] * p[AMPS_SN_L];
+        d.k[SNK_HALF_KE] = 0.5 * p[AMPS_SN_KE];
+    }
+    for (int i = 0; i < 24; ++i) d.kf[i] = (float)d.k[i];
+}
+
+bool has_low(const amps_ctx* c) { return c->mode != AMPS_PURE_FP64; }
+
+StepArgs step_args(amps_ctx* c, int from) {
+    StepArgs a{};
+    a.inst = c->d_desc;
+    a.tile_inst = c->d_tile_inst;
+    a.worklist = nullptr;
+    a.wl_count = nullptr;
+    a.wl_low = nullptr;
+    a.wl_low_count = nullptr;
+    if (c->model == AMPS_SNOWFLAKE && has_low(c)) {
+        a.worklist = c->d_wl;
+        a.wl_count = c->d_wl_count;
+        a.wl_low = c->d_wl + c->total_tiles;
+        a.wl_low_count = c->d_wl_count + 1;
+    }
+    a.type = has_low(c) ? c->d_type : nullptr;
+    a.total_cells = c->total_cells;
+    a.total_tiles = c->total_tiles;
+    a.nwork = c->total_tiles;
+    a.nfields = c->nfields;
+    a.q = c->q;
+    a.step_ctr = c->d_step;
+    a.step_add = 0;
+    a.in = c->buf[from];
+    a.out = c->buf[from ^ 1];
+    a.aux_in = c->d_aux[from];
+    a.aux_out = c->d_aux[from ^ 1];
+    return a;
+}
+
+int kev_flush(amps_ctx* c) {
+    if (c->kev_used == 0) return AMPS_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < c->kev_used; ++i) {
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, c->kev[2 * i], c->kev[2 * i + 1]));
+        c->kms += ms;
+    }
+    c->klaunches += c->kev_used;
+    c->kev_used = 0;
+    return AMPS_OK;
+}
+
+int launch_step_kernel(amps_ctx* c, const StepArgs& a) {
+    const bool adaptive = has_low(c);
+    const int grid = a.nwork;
+    if (c->model == AMPS_SNOWFLAKE)
+        return launch_sf_step(a, c->tile, adaptive, adaptive ? c->grid_hi : std::min(grid, c->grid_hi), c->grid_lo,
+                              c->stream);
+    if (c->model == AMPS_GRAINGROWTH) return launch_gg_step(a, c->tile, adaptive, grid, c->stream);
+    return launch_sn_step(a, c->tile, adaptive, grid, c->stream);
+}
+
+// Enqueue one explicit-Euler step reading buffer `from`.
+int enqueue_step(amps_ctx* c, int from, int step_add, bool capturing = false) {
+    StepArgs a = step_args(c, from);
+    a.step_add = step_add;
+    if (!c->ktiming || capturing) return launch_step_kernel(c, a);
+    if (c->kev_used * 2 + 2 > (int)c->kev.size()) RCCHK(kev_flush(c));
+    HIPCHK(hipEventRecord(c->kev[2 * c->kev_used], c->stream));
+    RCCHK(launch_step_kernel(c, a));
+    HIPCHK(hipEventRecord(c->kev[2 * c->kev_used + 1], c->stream));
+    c->kev_used++;
+    return AMPS_OK;
+}
+
+// Model auxiliary state that must follow the fields (grain growth: the
+// per-cell sum of squares, recomputed from current storage).
+int refresh_aux(amps_ctx* c, int b) {
+    if (c->model != AMPS_GRAINGROWTH) return AMPS_OK;
+    StepArgs a = step_args(c, b);
+    return launch_gg_aux(a, c->tile, c->stream);
+}
+
+// Rebuild the FP64 / low tile work-lists from the current types (snowflake).
+int refresh_worklists(amps_ctx* c) {
+    if (c->model != AMPS_SNOWFLAKE || !has_low(c)) return AMPS_OK;
+    return launch_worklist(c->d_type, c->total_tiles, c->d_wl, c->d_wl + c->total_tiles, c->d_wl_count, c->stream);
+}
+
+// One monitor pass on buffer `b`: monitor values, classification, conversion.
+int enqueue_monitor(amps_ctx* c, int b) {
+    StepArgs a = step_args(c, b);
+    int ngroups_eval = 1, field0 = c->mfield, stride = 0, broadcast = 0;
+    if (c->model == AMPS_GRAINGROWTH) {
+        if (c->mfield < 0) { ngroups_eval = c->q; field0 = 0; stride = 1; }
+        else broadcast = 1;
+    }
+    // the monitored field's own precision decides how it is read
+    if (c->gm.fg[field0] < 0) a.type = nullptr;
+    RCCHK(launch_fill_u64(c->d_stats + 1, 1, 0ull, c->stream));
+    RCCHK(launch_monitor(a, c->monitor, c->tile, ngroups_eval, field0, stride, c->ngroups, c->d_md, c->d_last, c->stream));
+    RCCHK(launch_classify(c->d_desc, c->d_tile_inst, c->total_tiles, c->ngroups, broadcast, c->d_md, c->thr_eff,
+                          c->d_type_old, c->d_type, c->d_stats, c->stream));
+    RCCHK(launch_convert(c->d_desc, c->d_tile_inst, c->buf[b], c->total_cells, c->total_tiles, c->d_type_old,
+                         c->d_type, c->gm, CONVERT_CHANGED, c->tile, c->stream));
+    if (c->model == AMPS_GRAINGROWTH) RCCHK(refresh_aux(c, b));
+    RCCHK(refresh_worklists(c));
+    return AMPS_OK;
+}
+
+int flush_adv(amps_ctx* c) {
+    if (c->pending_adv) {
+        RCCHK(launch_advance_step(c->d_step, c->pending_adv, c->stream));
+        c->pending_adv = 0;
+    }
+    return AMPS_OK;
+}
+
+// Steps whose index is a multiple of `every` start with a monitor pass.
+bool monitor_due(const amps_ctx* c, long long step) {
+    return c->mode == AMPS_ADAPTIVE && step % c->every == 0;
+}
+
+int enqueue_one(amps_ctx* c) {
+    if (monitor_due(c, c->steps_done)) {
+        RCCHK(enqueue_monitor(c, c->cur));
+        c->monitor_passes++;
+    }
+    RCCHK(enqueue_step(c, c->cur, c->pending_adv));
+    c->pending_adv++;
+    c->cur ^= 1;
+    c->steps_done++;
+    if (c->taping && c->tape_every > 0 && (c->steps_done - c->tape_step0) % c->tape_every == 0) {
+        RCCHK(flush_adv(c));
+        RCCHK(tape_push(c));
+    }
+    return AMPS_OK;
+}
+
+// Capture `period` steps (monitor pass included) starting from buffer parity p.
+int build_graph(amps_ctx* c, int p) {
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int b = p;
+    int rc = AMPS_OK;
+    for (int k = 0; k < c->graph_period && rc == AMPS_OK; ++k) {
+        if (c->mode == AMPS_ADAPTIVE && k % c->every == 0) rc = enqueue_monitor(c, b);
+        if (rc == AMPS_OK) rc = enqueue_step(c, b, k, true);
+        b ^= 1;
+    }
+    if (rc == AMPS_OK) rc = launch_advance_step(c->d_step, c->graph_period, c->stream);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc != AMPS_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return fail(AMPS_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->gexec[p], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(AMPS_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    return AMPS_OK;
+}
+
+}  // namespace
+
+// ============================================================ tape helpers
+int tape_push(amps_ctx* c) {
+    const size_t bytes = (size_t)c->nfields * c->total_cells * sizeof(double);
+    double* p = nullptr;
+    hipError_t e = hipMalloc((void**)&p, bytes);
+    if (e != hipSuccess) return fail(AMPS_E_NOMEM, "checkpoint allocation failed");
+    e = hipMemcpyAsync(p, c->buf[c->cur].f64, bytes, hipMemcpyDeviceToDevice, c->stream);
+    if (e != hipSuccess) { (void)hipFree(p); return fail(AMPS_E_HIP, "checkpoint copy failed"); }
+    c->tape.push_back(p);
+    c->tape_steps.push_back(c->steps_done);
+    return AMPS_OK;
+}
+
+namespace {
+void tape_clear(amps_ctx* c) {
+    for (double* p : c->tape) (void)hipFree(p);
+    c->tape.clear();
+    c->tape_steps.clear();
+    c->taping = false;
+}
+}  // namespace
+
+// ================================================================ C ABI
+extern "C" {
+
+const char* amps_status_string(int s) {
+    switch (s) {
+        case AMPS_OK: return "ok";
+        case AMPS_E_ARG: return "invalid argument";
+        case AMPS_E_NOMEM: return "out of memory";
+        case AMPS_E_HIP: return "HIP runtime error";
+        case AMPS_E_PARSE: return "parameter file parse error";
+        case AMPS_E_NONFINITE: return "non-finite value in a field";
+        case AMPS_E_STATE: return "call not valid in this state";
+        case AMPS_E_UNSUPPORTED: return "unsupported configuration";
+        case AMPS_E_IO: return "I/O error";
+        default: return "unknown status";
+    }
+}
+
+const char* amps_last_error(void) { return g_last_error.c_str(); }
+
+int amps_default_config(int model, amps_config* cfg) {
+    if (!cfg) return fail(AMPS_E_ARG, "cfg is NULL");
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->model = model;
+    cfg->mode = AMPS_PURE_FP64;
+    cfg->monitor = AMPS_MON_TEMPORAL;
+    cfg->nx = cfg->ny = 512;
+    cfg->q = 2;
+    cfg->tile = 16;
+    cfg->monitor_every = 10;
+    cfg->monitor_field = 0;
+    cfg->stencil = AMPS_STENCIL_REF;
+    cfg->bc = AMPS_BC_PERIODIC;
+    cfg->init = AMPS_INIT_REF;
+    cfg->threshold = 0.99;
+    cfg->seed = 1;
+    double* p = cfg->params;
+    if (model == AMPS_SNOWFLAKE) {  // snowflake/paras/para1.h:4-15, total.h:47
+        p[AMPS_SF_DX] = 0.03; p[AMPS_SF_DY] = 0.03; p[AMPS_SF_DT] = 1.0e-4; p[AMPS_SF_TAU] = 0.0003;
+        p[AMPS_SF_EPSB] = 0.01; p[AMPS_SF_KAPPA] = 1.8; p[AMPS_SF_DELTA] = 0.02; p[AMPS_SF_ANISO] = 4.0;
+        p[AMPS_SF_ALPHA] = 0.9; p[AMPS_SF_GAMMA] = 10.0; p[AMPS_SF_TEQ] = 1.0; p[AMPS_SF_THETA0] = 0.0;
+        p[AMPS_SF_PI] = 3.1415926; p[AMPS_SF_NOISE] = 0.0; p[AMPS_SF_R2] = 10.0;
+        p[AMPS_SF_CX] = 512 / 2 + 8; p[AMPS_SF_CY] = 512 / 2 + 8;
+    } else if (model == AMPS_GRAINGROWTH) {  // graingrowth/paras/para1.h:4-6
+        p[AMPS_GG_DX] = 0.5; p[AMPS_GG_DY] = 0.5; p[AMPS_GG_DT] = 0.005; p[AMPS_GG_L] = 5.0;
+        p[AMPS_GG_KAPPA] = 0.1; p[AMPS_GG_ALPHA] = 1.0; p[AMPS_GG_BETA] = 1.0; p[AMPS_GG_GAMMA] = 1.0;
+        p[AMPS_GG_R] = 50.0; p[AMPS_GG_CX] = 256.0; p[AMPS_GG_CY] = 256.0; p[AMPS_GG_AMP] = 0.0;
+        cfg->monitor_field = 1;  // the reference monitors eta2 (graingrowth/end2end_mix.cu:83)
+    } else if (model == AMPS_SINTERING) {  // sintering/paras/para1.h:1-9, function.h:72
+        p[AMPS_SN_DX] = 0.5; p[AMPS_SN_DY] = 0.5; p[AMPS_SN_DT] = 1.0e-4; p[AMPS_SN_A] = 16.0;
+        p[AMPS_SN_B] = 1.0; p[AMPS_SN_KC] = 5.0; p[AMPS_SN_KE] = 2.0; p[AMPS_SN_L] = 5.0;
+        p[AMPS_SN_DVOL] = 0.040; p[AMPS_SN_DVAP] = 0.002; p[AMPS_SN_DSURF] = 32.0; p[AMPS_SN_DGB] = 1.6;
+        p[AMPS_SN_R1] = 50.0; p[AMPS_SN_R2] = 25.0; p[AMPS_SN_X1] = 264.0; p[AMPS_SN_Y1] = 240.0;
+        p[AMPS_SN_X2] = 264.0; p[AMPS_SN_Y2] = 315.0; p[AMPS_SN_WIDTH] = 0.0; p[AMPS_SN_NOISE] = 0.0;
+        cfg->q = 2;
+    } else {
+        return fail(AMPS_E_ARG, "unknown model");
+    }
+    return AMPS_OK;
+}
+
+int amps_create_batch(const amps_config* cfgs, int n, amps_ctx** out) {
+    if (!cfgs || !out || n < 1 || n > AMPS_MAX_INST) return fail(AMPS_E_ARG, "need 1..256 configs and an output pointer");
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        RCCHK(validate(cfgs[i]));
+        const amps_config& a = cfgs[i];
+        const amps_config& b = cfgs[0];
+        if (a.model != b.model || a.mode != b.mode || a.tile != b.tile || a.monitor != b.monitor ||
+            a.monitor_every != b.monitor_every || a.monitor_field != b.monitor_field || a.threshold != b.threshold ||
+            (a.model == AMPS_GRAINGROWTH && a.q != b.q))
+            return fail(AMPS_E_ARG, "batched instances must share model, mode, tile, monitor settings and q");
+    }
+    amps_ctx* c = new amps_ctx();
+    c->cfg.assign(cfgs, cfgs + n);
+    c->ninst = n;
+    c->model = cfgs[0].model;
+    c->mode = cfgs[0].mode;
+    c->monitor = cfgs[0].monitor;
+    c->tile = cfgs[0].tile;
+    c->every = cfgs[0].monitor_every > 0 ? cfgs[0].monitor_every : 10;
+    c->mfield = cfgs[0].monitor_field;
+    c->q = c->model == AMPS_GRAINGROWTH ? cfgs[0].q : (c->model == AMPS_SINTERING ? 2 : 0);
+    c->nfields = nfields_of(cfgs[0]);
+    c->ngroups = c->model == AMPS_GRAINGROWTH ? c->q : 1;
+    c->thr_eff = c->monitor == AMPS_MON_GRADIENT ? cfgs[0].threshold * cfgs[0].threshold : cfgs[0].threshold;
+    c->gm.nfields = c->nfields;
+    c->gm.ngroups = c->ngroups;
+    for (int f = 0; f < AMPS_MAX_FIELDS; ++f) c->gm.fg[f] = -1;
+    for (int f = 0; f < c->nfields; ++f) {
+        if (c->model == AMPS_SNOWFLAKE) c->gm.fg[f] = 0;
+        else if (c->model == AMPS_GRAINGROWTH) c->gm.fg[f] = (signed char)f;
+        else c->gm.fg[f] = f == 0 ? -1 : 0;  // sintering: c stays FP64
+    }
+    if (c->mode == AMPS_ADAPTIVE && c->model == AMPS_GRAINGROWTH && c->mfield < 0) { /* per-field types */ }
+
+    c->desc.resize(n);
+    long long cells = 0;
+    int tiles = 0;
+    for (int i = 0; i < n; ++i) {
+        InstDesc& d = c->desc[i];
+        std::memset(&d, 0, sizeof(d));
+        d.nx = cfgs[i].nx;
+        d.ny = cfgs[i].ny;
+        d.tx = d.nx / c->tile;
+        d.ty = d.ny / c->tile;
+        d.tile_base = tiles;
+        d.bc = cfgs[i].bc;
+        d.stencil = cfgs[i].stencil;
+        d.cell_base = cells;
+        d.seed = cfgs[i].seed;
+        derive(d, cfgs[i]);
+        cells += (long long)d.nx * d.ny;
+        tiles += d.tx * d.ty;
+        c->max_inst_cells = std::max(c->max_inst_cells, (long long)d.nx * d.ny);
+        for (int t = 0; t < d.tx * d.ty; ++t) c->tile_inst.push_back(i);
+    }
+    c->total_cells = cells;
+    c->total_tiles = tiles;
+
+    int rc = AMPS_OK;
+    auto A = [&](void** p, size_t bytes) { if (rc == AMPS_OK) rc = dmalloc(c, p, bytes); };
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete c; return fail(AMPS_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        amps_destroy(c);
+        return fail(AMPS_E_HIP, "hipEventCreate failed");
+    }
+    A((void**)&c->d_desc, sizeof(InstDesc) * n);
+    A((void**)&c->d_tile_inst, sizeof(int) * tiles);
+    const size_t nf = (size_t)c->nfields;
+    for (int b = 0; b < 2; ++b) {
+        A((void**)&c->buf[b].f64, nf * cells * sizeof(double));
+        if (has_low(c)) {
+            A((void**)&c->buf[b].f16, nf * cells * sizeof(__half));
+            A((void**)&c->buf[b].off, nf * tiles * sizeof(double));
+            A((void**)&c->buf[b].scl, nf * tiles * sizeof(float));
+        }
+        if (c->model == AMPS_GRAINGROWTH) A((void**)&c->d_aux[b], cells * sizeof(double));
+    }
+    const size_t gt = (size_t)c->ngroups * tiles;
+    A((void**)&c->d_type, gt * sizeof(int));
+    A((void**)&c->d_type_old, gt * sizeof(int));
+    A((void**)&c->d_md, gt * sizeof(double));
+    if (c->mode == AMPS_ADAPTIVE && c->monitor == AMPS_MON_TEMPORAL) A((void**)&c->d_last, gt * 16 * sizeof(double));
+    A((void**)&c->d_stats, 4 * sizeof(unsigned long long));
+    A((void**)&c->d_step, sizeof(long long));
+    A((void**)&c->d_scratch, (size_t)c->max_inst_cells * sizeof(double));
+    A((void**)&c->d_wl, 2 * (size_t)tiles * sizeof(int));
+    A((void**)&c->d_wl_count, 2 * sizeof(int));
+    if (rc != AMPS_OK) { std::string m = g_last_error; amps_destroy(c); return fail(rc, m); }
+    e = hipMemcpy(c->d_desc, c->desc.data(), sizeof(InstDesc) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->d_tile_inst, c->tile_inst.data(), sizeof(int) * tiles, hipMemcpyHostToDevice);
+    if (e != hipSuccess) { amps_destroy(c); return fail(AMPS_E_HIP, std::string("upload: ") + hipGetErrorString(e)); }
+    int cus = 256;
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) cus = v;
+    }
+    // persistent grids: resident workgroups per CU of each sweep kernel
+    c->grid_hi = std::min(tiles, cus * 3);
+    c->grid_lo = std::min(tiles, cus * 5);
+    const char* ng = std::getenv("AMPS_NO_GRAPH");
+    c->use_graph = !(ng && ng[0] == '1');
+    c->graph_period = c->mode == AMPS_ADAPTIVE ? (c->every % 2 ? 2 * c->every : c->every) : 10;
+    *out = c;
+    return AMPS_OK;
+}
+
+int amps_create(const amps_config* cfg, amps_ctx** out) { return amps_create_batch(cfg, 1, out); }
+
+void amps_destroy(amps_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    tape_clear(c);
+    for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
+    for (int p = 0; p < 2; ++p)
+        if (c->gexec[p]) (void)hipGraphExecDestroy(c->gexec[p]);
+    for (void* p : c->allocs) (void)hipFree(p);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int amps_init(amps_ctx* c) {
+    if (!c) return fail(AMPS_E_ARG, "ctx is NULL");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    tape_clear(c);
+    c->cur = 0;
+    for (int i = 0; i < c->ninst; ++i)
+        RCCHK(launch_init(c->d_desc, i, (long long)c->desc[i].nx * c->desc[i].ny, c->model, c->cfg[i].init, c->q,
+                          c->buf[0], c->total_cells, c->stream));
+    std::vector<int> types((size_t)c->ngroups * c->total_tiles, c->mode == AMPS_PURE_HALF2 ? 1 : 2);
+    HIPCHK(hipMemcpyAsync(c->d_type, types.data(), types.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_type_old, types.data(), types.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    long long zero = 0;
+    HIPCHK(hipMemcpyAsync(c->d_step, &zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
+    unsigned long long z4[4] = {0, 0, 0, 0};
+    HIPCHK(hipMemcpyAsync(c->d_stats, z4, sizeof(z4), hipMemcpyHostToDevice, c->stream));
+    if (c->mode == AMPS_PURE_HALF2)
+        RCCHK(launch_convert(c->d_desc, c->d_tile_inst, c->buf[0], c->total_cells, c->total_tiles, nullptr, c->d_type,
+                             c->gm, CONVERT_ENCODE_LOW, c->tile, c->stream));
+    if (c->mode == AMPS_ADAPTIVE && c->monitor == AMPS_MON_TEMPORAL) {
+        StepArgs a = step_args(c, 0);
+        int ngroups_eval = 1, field0 = c->mfield, stride = 0;
+        if (c->model == AMPS_GRAINGROWTH && c->mfield < 0) { ngroups_eval = c->q; field0 = 0; stride = 1; }
+        RCCHK(launch_monitor_last_init(a, c->tile, ngroups_eval, field0, stride, c->d_last, c->stream));
+    }
+    RCCHK(refresh_aux(c, 0));
+    RCCHK(refresh_worklists(c));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->steps_done = 0;
+    c->monitor_passes = 0;
+    c->pending_adv = 0;
+    c->timing_open = false;
+    c->initialised = true;
+    return AMPS_OK;
+}
+
+int amps_step(amps_ctx* c, int nsteps) {
+    if (!c) return fail(AMPS_E_ARG, "ctx is NULL");
+    if (!c->initialised) return fail(AMPS_E_STATE, "amps_init has not been called");
+    if (nsteps < 0) return fail(AMPS_E_ARG, "nsteps < 0");
+    if (nsteps == 0) return AMPS_OK;
+    if (!c->timing_open) {
+        HIPCHK(hipEventRecord(c->ev0, c->stream));
+        c->timing_open = true;
+    }
+    int left = nsteps;
+    const bool graphs = c->use_graph && !c->taping && !c->ktiming;
+    while (left > 0) {
+        const long long s = c->steps_done;
+        if (graphs && left >= c->graph_period && s % c->graph_period == 0) {
+            RCCHK(flush_adv(c));
+            if (!c->gexec[c->cur]) RCCHK(build_graph(c, c->cur));
+            HIPCHK(hipGraphLaunch(c->gexec[c->cur], c->stream));
+            // graph_period is even: the buffer parity is unchanged
+            c->steps_done += c->graph_period;
+            if (c->mode == AMPS_ADAPTIVE) c->monitor_passes += c->graph_period / c->every;
+            left -= c->graph_period;
+        } else {
+            RCCHK(enqueue_one(c));
+            left--;
+        }
+    }
+    RCCHK(flush_adv(c));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    return AMPS_OK;
+}
+
+int amps_sync(amps_ctx* c, double* elapsed_ms) {
+    if (!c) return fail(AMPS_E_ARG, "ctx is NULL");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (elapsed_ms) *elapsed_ms = 0.0;
+    if (c->timing_open) {
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        if (elapsed_ms) *elapsed_ms = ms;
+        c->timing_open = false;
+    }
+    return AMPS_OK;
+}
+
+int amps_monitor_now(amps_ctx* c) {
+    if (!c || !c->initialised) return fail(AMPS_E_STATE, "context not initialised");
+    if (c->mode != AMPS_ADAPTIVE) return fail(AMPS_E_STATE, "monitor passes need ADAPTIVE mode");
+    RCCHK(enqueue_monitor(c, c->cur));
+    c->monitor_passes++;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return AMPS_OK;
+}
+
+int amps_get_field(amps_ctx* c, int inst, int field, double* dst, size_t n) {
+    if (!c || !dst) return fail(AMPS_E_ARG, "NULL argument");
+    if (inst < 0 || inst >= c->ninst || field < 0 || field >= c->nfields) return fail(AMPS_E_ARG, "instance or field out of range");
+    const InstDesc& d = c->desc[inst];
+    if (n != (size_t)d.nx * d.ny) return fail(AMPS_E_ARG, "n must equal nx*ny of the instance");
+    const int group = has_low(c) ? c->gm.fg[field] : -1;
+    RCCHK(launch_gather(c->d_desc, c->buf[c->cur], c->total_cells, c->total_tiles, has_low(c) ? c->d_type : nullptr,
+                        group, inst, d.tx * d.ty, field, c->d_scratch, c->tile, c->stream));
+    HIPCHK(hipMemcpyAsync(dst, c->d_scratch, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(dst[i])) return fail(AMPS_E_NONFINITE, "field " + std::to_string(field) + " holds non-finite values");
+    return AMPS_OK;
+}
+
+int amps_set_field(amps_ctx* c, int inst, int field, const double* src, size_t n) {
+    if (!c || !src) return fail(AMPS_E_ARG, "NULL argument");
+    if (inst < 0 || inst >= c->ninst || field < 0 || field >= c->nfields) return fail(AMPS_E_ARG, "instance or field out of range");
+    const InstDesc& d = c->desc[inst];
+    if (n != (size_t)d.nx * d.ny) return fail(AMPS_E_ARG, "n must equal nx*ny of the instance");
+    double* dstp = c->buf[c->cur].f64 + (size_t)field * c->total_cells + d.cell_base;
+    HIPCHK(hipMemcpyAsync(dstp, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (has_low(c) && c->gm.fg[field] >= 0) {
+        GroupMap one = c->gm;
+        for (int f = 0; f < c->nfields; ++f) if (f != field) one.fg[f] = -2;
+        RCCHK(launch_convert(c->d_desc, c->d_tile_inst, c->buf[c->cur], c->total_cells, c->total_tiles, nullptr,
+                             c->d_type, one, CONVERT_ENCODE_LOW, c->tile, c->stream));
+    }
+    RCCHK(refresh_aux(c, c->cur));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return AMPS_OK;
+}
+
+int amps_get_tile_types(amps_ctx* c, int inst, int32_t* dst, size_t n) {
+    if (!c || !dst) return fail(AMPS_E_ARG, "NULL argument");
+    if (inst < 0 || inst >= c->ninst) return fail(AMPS_E_ARG, "instance out of range");
+    const InstDesc& d = c->desc[inst];
+    const size_t nt = (size_t)d.tx * d.ty;
+    if (n != nt * c->ngroups) return fail(AMPS_E_ARG, "n must equal tiles * type_groups");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int g = 0; g < c->ngroups; ++g)
+        HIPCHK(hipMemcpy(dst + g * nt, c->d_type + (size_t)g * c->total_tiles + d.tile_base, nt * sizeof(int),
+                         hipMemcpyDeviceToHost));
+    return AMPS_OK;
+}
+
+int amps_set_tile_types(amps_ctx* c, int inst, const int32_t* src, size_t n) {
+    if (!c || !src) return fail(AMPS_E_ARG, "NULL argument");
+    if (inst < 0 || inst >= c->ninst) return fail(AMPS_E_ARG, "instance out of range");
+    if (!has_low(c)) return fail(AMPS_E_STATE, "PURE_FP64 contexts have no tile map");
+    const InstDesc& d = c->desc[inst];
+    const size_t nt = (size_t)d.tx * d.ty;
+    if (n != nt * c->ngroups) return fail(AMPS_E_ARG, "n must equal tiles * type_groups");
+    for (size_t i = 0; i < n; ++i)
+        if (src[i] != 1 && src[i] != 2) return fail(AMPS_E_ARG, "tile types must be 1 or 2");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(c->d_type_old, c->d_type, (size_t)c->ngroups * c->total_tiles * sizeof(int), hipMemcpyDeviceToDevice));
+    for (int g = 0; g < c->ngroups; ++g)
+        HIPCHK(hipMemcpy(c->d_type + (size_t)g * c->total_tiles + d.tile_base, src + g * nt, nt * sizeof(int),
+                         hipMemcpyHostToDevice));
+    RCCHK(launch_convert(c->d_desc, c->d_tile_inst, c->buf[c->cur], c->total_cells, c->total_tiles, c->d_type_old,
+                         c->d_type, c->gm, CONVERT_CHANGED, c->tile, c->stream));
+    RCCHK(refresh_aux(c, c->cur));
+    RCCHK(refresh_worklists(c));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return AMPS_OK;
+}
+
+int amps_get_monitor_values(amps_ctx* c, int inst, double* dst, size_t n) {
+    if (!c || !dst) return fail(AMPS_E_ARG, "NULL argument");
+    if (inst < 0 || inst >= c->ninst) return fail(AMPS_E_ARG, "instance out of range");
+    const InstDesc& d = c->desc[inst];
+    const size_t nt = (size_t)d.tx * d.ty;
+    if (n != nt * c->ngroups) return fail(AMPS_E_ARG, "n must equal tiles * type_groups");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int g = 0; g < c->ngroups; ++g)
+        HIPCHK(hipMemcpy(dst + g * nt, c->d_md + (size_t)g * c->total_tiles + d.tile_base, nt * sizeof(double),
+                         hipMemcpyDeviceToHost));
+    return AMPS_OK;
+}
+
+int amps_get_info(amps_ctx* c, amps_info* info) {
+    if (!c || !info) return fail(AMPS_E_ARG, "NULL argument");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    unsigned long long st[4];
+    HIPCHK(hipMemcpy(st, c->d_stats, sizeof(st), hipMemcpyDeviceToHost));
+    std::memset(info, 0, sizeof(*info));
+    info->steps_done = c->steps_done;
+    info->monitor_passes = c->monitor_passes;
+    int maxt = 0;
+    for (auto& d : c->desc) maxt = std::max(maxt, d.tx * d.ty);
+    info->ntiles = maxt;
+    info->type_groups = c->ngroups;
+    info->nfields = c->nfields;
+    info->ninstances = c->ninst;
+    const double denom = (double)c->ngroups * c->total_tiles;
+    if (c->mode == AMPS_ADAPTIVE && c->monitor_passes > 0) {
+        info->hi_frac_last = st[1] / denom;
+        info->hi_frac_mean = st[0] / (denom * c->monitor_passes);
+    } else {
+        std::vector<int> t((size_t)c->ngroups * c->total_tiles);
+        if (has_low(c)) HIPCHK(hipMemcpy(t.data(), c->d_type, t.size() * sizeof(int), hipMemcpyDeviceToHost));
+        size_t hi = 0;
+        for (int v : t) hi += (v == 2) || !has_low(c);
+        info->hi_frac_last = info->hi_frac_mean = has_low(c) ? hi / denom : 1.0;
+    }
+    info->cells = (double)c->total_cells;
+    return AMPS_OK;
+}
+
+}  // extern "C"
+
+// ======================================================= kernel timing
+extern "C" int amps_set_kernel_timing(amps_ctx* c, int enable) {
+    if (!c) return fail(AMPS_E_ARG, "ctx is NULL");
+    RCCHK(kev_flush(c));
+    if (enable && c->kev.empty()) {
+        c->kev.resize(2 * 512);
+        for (auto& e : c->kev) HIPCHK(hipEventCreate(&e));
+    }
+    c->ktiming = enable != 0;
+    c->klaunches = 0;
+    c->kms = 0.0;
+    return AMPS_OK;
+}
+
+extern "C" int amps_get_kernel_time(amps_ctx* c, int64_t* launches, double* total_ms) {
+    if (!c || !launches || !total_ms) return fail(AMPS_E_ARG, "NULL argument");
+    RCCHK(kev_flush(c));
+    *launches = c->klaunches;
+    *total_ms = c->kms;
+    return AMPS_OK;
+}
+
+// ================================================================ adjoint
+extern "C" int amps_tape_begin(amps_ctx* c, int ckpt_every) {
+    (void)c; (void)ckpt_every;
+    return fail(AMPS_E_UNSUPPORTED, "adjoint not built yet");
+}
+
+extern "C" int amps_adjoint(amps_ctx* c, const double* dJ_du, const int32_t* param_ids, int np, double* dJ_dp,
+                            double* dJ_du0) {
+    (void)c; (void)dJ_du; (void)param_ids; (void)np; (void)dJ_dp; (void)dJ_du0;
+    return fail(AMPS_E_UNSUPPORTED, "adjoint not built yet");
+}

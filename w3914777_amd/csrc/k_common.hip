@@ -1,0 +1,482 @@
+// k_common.hip — model-independent kernels: field initialisation, the
+// per-tile precision monitor, tile classification and FP64 <-> fp16 tile
+// conversion, plus decode/encode used by get/set of fields.
+#include "kernels.hpp"
+#include "tile_common.hpp"
+
+namespace amps {
+
+// ================================================================== init
+// snowflake/tools.h:24-33 (disc, T = 0); graingrowth/tools.h:24-38 (circle
+// grain) or seeded U[-amp, amp]; sintering/main.cu:45-60 (1-based discs) or
+// tanh discs plus seeded noise.  One thread per cell, every field of one
+// instance; writes FP64 storage of buffer `b`.
+__global__ void init_kernel(const InstDesc* inst_all, int inst, int model, int init, int q, Buf b,
+                            long long total_cells) {
+    const InstDesc& d = inst_all[inst];
+    const long long n = (long long)d.nx * d.ny;
+    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n;
+         c += (long long)gridDim.x * blockDim.x) {
+        const int y = (int)(c / d.nx), x = (int)(c - (long long)y * d.nx);
+        double* f = b.f64 + d.cell_base + c;
+        if (model == AMPS_SNOWFLAKE) {
+            double dx = x - d.p[AMPS_SF_CX], dy = y - d.p[AMPS_SF_CY];
+            f[0] = (dx * dx + dy * dy < d.p[AMPS_SF_R2]) ? 1.0 : 0.0;
+            f[total_cells] = 0.0;
+        } else if (model == AMPS_GRAINGROWTH) {
+            if (init == AMPS_INIT_REF) {
+                double dx = x - d.p[AMPS_GG_CX], dy = y - d.p[AMPS_GG_CY], R = d.p[AMPS_GG_R];
+                bool in = dx * dx + dy * dy <= R * R;
+                for (int i = 0; i < q; ++i) f[i * total_cells] = 0.0;
+                f[0] = in ? 0.0 : 1.0;
+                f[total_cells] = in ? 1.0 : 0.0;
+            } else {
+                double amp = d.p[AMPS_GG_AMP];
+                for (int i = 0; i < q; ++i) {
+                    uint64_t key = rng_key(d.seed, STREAM_GG_INIT + (uint64_t)i);
+                    f[i * total_cells] = amp * (2.0 * rng_u01_keyed(key, (uint64_t)c) - 1.0);
+                }
+            }
+        } else {
+            const double R1 = d.p[AMPS_SN_R1], R2 = d.p[AMPS_SN_R2];
+            const double X1 = d.p[AMPS_SN_X1], Y1 = d.p[AMPS_SN_Y1], X2 = d.p[AMPS_SN_X2], Y2 = d.p[AMPS_SN_Y2];
+            double cv, e1, e2;
+            if (init == AMPS_INIT_REF) {
+                const int xx = x + 1, yy = y + 1;  // reference loops are 1-based
+                cv = e1 = e2 = 0.0;
+                if ((xx - X1) * (xx - X1) + (yy - Y1) * (yy - Y1) <= R1 * R1) { cv = 1.0; e1 = 1.0; }
+                if ((xx - X2) * (xx - X2) + (yy - Y2) * (yy - Y2) <= R2 * R2) { cv = 1.0; e2 = 1.0; e1 = 0.0; }
+            } else {
+                const double W = d.p[AMPS_SN_WIDTH], a = d.p[AMPS_SN_NOISE];
+                double d1 = sqrt((x - X1) * (x - X1) + (y - Y1) * (y - Y1));
+                double d2 = sqrt((x - X2) * (x - X2) + (y - Y2) * (y - Y2));
+                e1 = 0.5 * (1.0 - tanh((d1 - R1) / W));
+                e2 = 0.5 * (1.0 - tanh((d2 - R2) / W));
+                uint64_t key = rng_key(d.seed, STREAM_SN_INIT);
+                cv = e1 + e2 + a * (2.0 * rng_u01_keyed(key, (uint64_t)c) - 1.0);
+                cv = cv < 0.0 ? 0.0 : (cv > 1.0 ? 1.0 : cv);
+            }
+            f[0] = cv;
+            f[total_cells] = e1;
+            f[2 * total_cells] = e2;
+        }
+    }
+}
+
+// ============================================================ fp16 codec
+// Direct double -> binary16 rounding to nearest even: round to odd into
+// float (24 bits), then to nearest even into half (11 bits); with at least
+// two extra bits the two-step result equals one correct rounding.
+__device__ inline __half f64_to_f16_rne(double x) {
+    float f = (float)x;
+    double back = (double)f;
+    if (back != x && !isnan(x)) {
+        uint32_t bits = __float_as_uint(f);
+        if ((bits & 1u) == 0u) bits += (fabs(back) < fabs(x)) ? 1u : (uint32_t)-1;
+        f = __uint_as_float(bits);
+    }
+    return __float2half_rn(f);
+}
+
+// Encode the FP64 values of tile `gtile`, field f of buffer b into fp16
+// (offset = mid-range, power-of-two scale).  Whole workgroup; barriers inside.
+template <int TS>
+__device__ void encode_tile(const Buf& b, long long total_cells, int total_tiles, const InstDesc& d, int gtile,
+                            int field, double* red) {
+    const int lt = gtile - d.tile_base, tyi = lt / d.tx, txi = lt - tyi * d.tx;
+    const double* f64 = b.f64 + (size_t)field * total_cells + d.cell_base;
+    __half* f16 = b.f16 + (size_t)field * total_cells + d.cell_base;
+    double mn = INFINITY, mx = -INFINITY;
+    for (int e = threadIdx.x; e < TS * TS; e += WG) {
+        int ly = e / TS, lx = e - ly * TS;
+        double v = f64[(size_t)(tyi * TS + ly) * d.nx + txi * TS + lx];
+        mn = fmin(mn, v);
+        mx = fmax(mx, v);
+    }
+    mx = block_max_d(mx, red);
+    __syncthreads();
+    mn = -block_max_d(-mn, red);
+    double off = 0.5 * (mn + mx), hw = 0.5 * (mx - mn);
+    int ex = 0;
+    if (hw > 0.0 && hw <= 1e300) frexp(hw, &ex);
+    if (ex < -100) ex = -100;
+    if (ex > 100) ex = 100;
+    const double inv = ldexp(1.0, 14 - ex), scl = ldexp(1.0, ex - 14);
+    double* w64 = const_cast<double*>(f64);
+    for (int e = threadIdx.x; e < TS * TS; e += WG) {
+        int ly = e / TS, lx = e - ly * TS;
+        size_t c = (size_t)(tyi * TS + ly) * d.nx + txi * TS + lx;
+        const __half h = f64_to_f16_rne((f64[c] - off) * inv);
+        f16[c] = h;
+        // FP64 storage keeps the decoded value: neighbours read halos there
+        w64[c] = off + (double)__half2float(h) * scl;
+    }
+    if (threadIdx.x == 0) {
+        b.off[(size_t)field * total_tiles + gtile] = off;
+        b.scl[(size_t)field * total_tiles + gtile] = ldexpf(1.0f, ex - 14);
+    }
+}
+
+template <int TS>
+__device__ void decode_tile(const Buf& b, long long total_cells, int total_tiles, const InstDesc& d, int gtile,
+                            int field, double* dst_f64 /* field base incl. instance */) {
+    const int lt = gtile - d.tile_base, tyi = lt / d.tx, txi = lt - tyi * d.tx;
+    const __half* f16 = b.f16 + (size_t)field * total_cells + d.cell_base;
+    const double off = b.off[(size_t)field * total_tiles + gtile];
+    const double scl = b.scl[(size_t)field * total_tiles + gtile];
+    for (int e = threadIdx.x; e < TS * TS; e += WG) {
+        int ly = e / TS, lx = e - ly * TS;
+        size_t c = (size_t)(tyi * TS + ly) * d.nx + txi * TS + lx;
+        dst_f64[c] = off + (double)__half2float(f16[c]) * scl;
+    }
+}
+
+// Convert tiles whose type changed (old -> new) for every field of the group:
+// 2 -> 1 encodes FP64 into fp16, 1 -> 2 decodes into FP64.  force_encode /
+// force_decode convert every tile whose new type is 1 / every type-1 tile.
+// One workgroup per (group, tile).
+template <int TS>
+__global__ __launch_bounds__(256) void convert_kernel(const InstDesc* inst, const int* tile_inst, Buf b,
+                                                       long long total_cells, int total_tiles, const int* type_old,
+                                                       const int* type, GroupMap gm, int mode) {
+    __shared__ double red[4];
+    const int ngw = gm.ngroups * total_tiles;
+    for (int w = blockIdx.x; w < ngw; w += gridDim.x) {
+        const int g = w / total_tiles, gtile = w - g * total_tiles;
+        const int tn = type[w];
+        const int to = type_old ? type_old[w] : tn;
+        bool enc = false, dec = false;
+        if (mode == CONVERT_CHANGED) { enc = (to == 2 && tn == 1); dec = (to == 1 && tn == 2); }
+        else if (mode == CONVERT_ENCODE_LOW) enc = (tn == 1);
+        else if (mode == CONVERT_DECODE_LOW) dec = (tn == 1);
+        if (!enc && !dec) continue;
+        const InstDesc& d = inst[tile_inst[gtile]];
+        for (int f = 0; f < gm.nfields; ++f) {
+            if (gm.fg[f] != g) continue;
+            if (enc) encode_tile<TS>(b, total_cells, total_tiles, d, gtile, f, red);
+            else decode_tile<TS>(b, total_cells, total_tiles, d, gtile, f, b.f64 + (size_t)f * total_cells + d.cell_base);
+            __syncthreads();
+        }
+    }
+}
+
+// ================================================================ monitor
+// Sample lattices of the reference (tile 16): {0,5,10,15} for Monitor1
+// (snowflake/tools.h:186), {2,6,10,14} for Monitor2 (tools.h:203); for
+// tile t: {0,(t-1)/3,2(t-1)/3,t-1} and {t/8,3t/8,5t/8,7t/8}.
+__device__ inline int sample_off(int tile, int kind, int i) {
+    if (kind == AMPS_MON_SPATIAL) return i == 0 ? 0 : (i == 1 ? (tile - 1) / 3 : (i == 2 ? 2 * (tile - 1) / 3 : tile - 1));
+    return (2 * i + 1) * tile / 8;
+}
+
+__device__ inline double read_decoded(const Buf& b, long long total_cells, int total_tiles, const InstDesc& d,
+                                      int field, int gtile, int ty, int y, int x) {
+    size_t c = (size_t)field * total_cells + d.cell_base + (size_t)y * d.nx + x;
+    if (ty == 2) return b.f64[c];
+    return b.off[(size_t)field * total_tiles + gtile] + (double)__half2float(b.f16[c]) *
+                                                         (double)b.scl[(size_t)field * total_tiles + gtile];
+}
+
+// Monitor1 / Monitor2 on the 4x4 sample lattice, one thread per (group,
+// tile).  Monitor1: get_max_diff1 (snowflake/tools.h:177-193), min starts
+// at 1 and max at 0.  Monitor2: get_max_diff2 (tools.h:194-211) against the
+// samples kept from the previous pass, which are then refreshed (the
+// reference refreshes phi_last right after the monitor, function.h:106-113).
+__global__ void monitor_samples_kernel(const InstDesc* inst, const int* tile_inst, Buf b, long long total_cells,
+                                       int total_tiles, const int* type, int ngroups_eval, int field0,
+                                       int field_stride, int type_groups, int tile, int kind, double* md,
+                                       double* last) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= ngroups_eval * total_tiles) return;
+    const int g = w / total_tiles, gtile = w - g * total_tiles;
+    const int field = field0 + g * field_stride;
+    const InstDesc& d = inst[tile_inst[gtile]];
+    const int lt = gtile - d.tile_base, tyi = lt / d.tx, txi = lt - tyi * d.tx;
+    const int tgroup = type_groups > 1 ? (g < type_groups ? g : 0) : 0;
+    const int ty = type ? type[(size_t)tgroup * total_tiles + gtile] : 2;
+    if (kind == AMPS_MON_SPATIAL) {
+        double mn = 1.0, mx = 0.0;
+        for (int j = 0; j < 4; ++j)
+            for (int i = 0; i < 4; ++i) {
+                double v = read_decoded(b, total_cells, total_tiles, d, field, gtile, ty,
+                                        tyi * tile + sample_off(tile, kind, j), txi * tile + sample_off(tile, kind, i));
+                if (v < mn) mn = v;
+                if (v > mx) mx = v;
+            }
+        md[w] = mx - mn;
+    } else {
+        double m = 0.0;
+        double* lw = last + (size_t)w * 16;
+        for (int j = 0; j < 4; ++j)
+            for (int i = 0; i < 4; ++i) {
+                double v = read_decoded(b, total_cells, total_tiles, d, field, gtile, ty,
+                                        tyi * tile + sample_off(tile, kind, j), txi * tile + sample_off(tile, kind, i));
+                double df = fabs(lw[j * 4 + i] - v);
+                if (df > m) m = df;
+                lw[j * 4 + i] = v;
+            }
+        md[w] = m;
+    }
+}
+
+// Monitor2 history at initialisation: the complement of the field
+// (phi_old = phi == 0 ? 1 : 0, snowflake/end2end_mix.cu:44-46).
+__global__ void monitor_last_init_kernel(const InstDesc* inst, const int* tile_inst, Buf b, long long total_cells,
+                                         int total_tiles, int ngroups_eval, int field0, int field_stride, int tile,
+                                         double* last) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= ngroups_eval * total_tiles) return;
+    const int g = w / total_tiles, gtile = w - g * total_tiles;
+    const int field = field0 + g * field_stride;
+    const InstDesc& d = inst[tile_inst[gtile]];
+    const int lt = gtile - d.tile_base, tyi = lt / d.tx, txi = lt - tyi * d.tx;
+    for (int j = 0; j < 4; ++j)
+        for (int i = 0; i < 4; ++i) {
+            double v = read_decoded(b, total_cells, total_tiles, d, field, gtile, 2,
+                                    tyi * tile + sample_off(tile, AMPS_MON_TEMPORAL, j),
+                                    txi * tile + sample_off(tile, AMPS_MON_TEMPORAL, i));
+            last[(size_t)w * 16 + j * 4 + i] = v == 0.0 ? 1.0 : 0.0;
+        }
+}
+
+// Gradient monitor: max over the tile of (0.5*(f[x+1]-f[x-1]))^2 +
+// (0.5*(f[y+1]-f[y-1])*dx/dy)^2, neighbours through the boundary map.
+// Contraction into FMA is off so the value matches a CPU evaluation bit for
+// bit.  One workgroup per (group, tile).
+template <int TS>
+__global__ __launch_bounds__(256) void monitor_grad_kernel(StepArgs a, int ngroups_eval, int field0,
+                                                           int field_stride, int type_groups, double* md) {
+    constexpr int H = 1, LW = TS + 2;
+    __shared__ double box[LW * LW];
+    __shared__ NbField nbf;
+    __shared__ double red[4];
+    const int ngw = ngroups_eval * a.total_tiles;
+    for (int w = blockIdx.x; w < ngw; w += gridDim.x) {
+        const int g = w / a.total_tiles, gtile = w - g * a.total_tiles;
+        const int field = field0 + g * field_stride;
+        const int tgroup = type_groups > 1 ? g : 0;
+        const InstDesc& d = a.inst[a.tile_inst[gtile]];
+        const int lt = gtile - d.tile_base, tyi = lt / d.tx, txi = lt - tyi * d.tx;
+        Nb nb;
+        nb_setup(nb, d, txi, tyi, TS);
+        const bool uni = (a.type == nullptr);
+        if (!uni) {
+            nbfield_fill(&nbf, a, a.in, nb, field, tgroup);
+            __syncthreads();
+        }
+        BoxMap<TS, H> bm;
+        box_map<TS, H>(bm, d, txi, tyi);
+        const double* f64 = a.in.f64 + (size_t)field * a.total_cells + d.cell_base;
+        if (uni) box_load1_f64<TS, H>(box, f64, bm);
+        else box_load_mixed_f64<TS, H>(box, f64, a.in.f16 + (size_t)field * a.total_cells + d.cell_base, &nbf, bm);
+        __syncthreads();
+        const double r = d.p[0] / d.p[1];  // dx / dy (slots 0 and 1 of every model)
+        double m = 0.0;
+        for (int e = threadIdx.x; e < TS * TS; e += WG) {
+#pragma clang fp contract(off)
+            int ly = e / TS, lx = e - ly * TS;
+            int c = (ly + 1) * LW + lx + 1;
+            double hx = 0.5 * (box[c + 1] - box[c - 1]);
+            double hy = 0.5 * (box[c + LW] - box[c - LW]) * r;
+            double qv = hx * hx + hy * hy;
+            if (qv > m) m = qv;
+        }
+        m = block_max_d(m, red);
+        if (threadIdx.x == 0) md[w] = m;
+        __syncthreads();
+    }
+}
+
+// get_type (snowflake/tools.h:212-230): type_old <- type; neighbour max over
+// self and the 4 non-periodic von-Neumann neighbours with fmax (CUDA's max);
+// < thr -> 1, >= thr -> 2, unordered keeps the old type.  One thread per
+// (group, tile); with broadcast every group takes group 0's monitor value.
+__global__ void classify_kernel(const InstDesc* inst, const int* tile_inst, int total_tiles, int ngroups,
+                                int broadcast, const double* md, double thr, int* type_old, int* type,
+                                unsigned long long* stats) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= ngroups * total_tiles) return;
+    const int g = w / total_tiles, gtile = w - g * total_tiles;
+    const InstDesc& d = inst[tile_inst[gtile]];
+    const int lt = gtile - d.tile_base, uy = lt / d.tx, ux = lt - uy * d.tx;
+    const double* m0 = md + (size_t)(broadcast ? 0 : g) * total_tiles;
+    double m = m0[gtile];
+    if (uy < d.ty - 1) m = fmax(m, m0[gtile + d.tx]);
+    if (uy > 0) m = fmax(m, m0[gtile - d.tx]);
+    if (ux < d.tx - 1) m = fmax(m, m0[gtile + 1]);
+    if (ux > 0) m = fmax(m, m0[gtile - 1]);
+    int t = type[w];
+    type_old[w] = t;
+    if (m < thr) t = 1;
+    else if (m >= thr) t = 2;
+    type[w] = t;
+    unsigned long long hb = __ballot(t == 2);
+    if ((threadIdx.x & 63) == 0 && hb) {
+        atomicAdd(&stats[0], (unsigned long long)__popcll(hb));
+        atomicAdd(&stats[1], (unsigned long long)__popcll(hb));
+    }
+}
+
+// Work-lists of FP64 (type 2) and low (type 1) tiles in tile order, built
+// by one workgroup with wave ballots and a block prefix (group 0 types).
+__global__ __launch_bounds__(1024) void worklist_kernel(const int* type, int total_tiles, int* wl_hi, int* wl_lo,
+                                                        int* counts) {
+    __shared__ int sh[16], sl[16];
+    __shared__ int base[2];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) base[0] = base[1] = 0;
+    __syncthreads();
+    for (int c0 = 0; c0 < total_tiles; c0 += 1024) {
+        const int t = c0 + tid;
+        const int ty = t < total_tiles ? type[t] : 0;
+        const bool hi = ty == 2, lo = ty == 1;
+        const unsigned long long bh = __ballot(hi), bl = __ballot(lo);
+        const unsigned long long below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+        if (lane == 0) { sh[wave] = __popcll(bh); sl[wave] = __popcll(bl); }
+        __syncthreads();
+        int oh = base[0], ol = base[1];
+        for (int w = 0; w < wave; ++w) { oh += sh[w]; ol += sl[w]; }
+        if (hi) wl_hi[oh + __popcll(bh & below)] = t;
+        if (lo) wl_lo[ol + __popcll(bl & below)] = t;
+        __syncthreads();
+        if (tid == 0) {
+            int a = 0, b = 0;
+            for (int w = 0; w < 16; ++w) { a += sh[w]; b += sl[w]; }
+            base[0] += a;
+            base[1] += b;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) { counts[0] = base[0]; counts[1] = base[1]; }
+}
+
+// Decode every low tile of field f into a dense FP64 array (instance order),
+// for amps_get_field.  FP64 tiles are copied.
+template <int TS>
+__global__ __launch_bounds__(256) void gather_kernel(const InstDesc* inst, Buf b, long long total_cells,
+                                                     int total_tiles, const int* type, int group, int instid,
+                                                     int field, double* dst) {
+    const InstDesc& d = inst[instid];
+    const int ntile = d.tx * d.ty;
+    for (int lt = blockIdx.x; lt < ntile; lt += gridDim.x) {
+        const int gtile = d.tile_base + lt;
+        const int ty = (type && group >= 0) ? type[(size_t)group * total_tiles + gtile] : 2;
+        const int tyi = lt / d.tx, txi = lt - tyi * d.tx;
+        if (ty == 1) {
+            decode_tile<TS>(b, total_cells, total_tiles, d, gtile, field, dst);
+        } else {
+            const double* src = b.f64 + (size_t)field * total_cells + d.cell_base;
+            for (int e = threadIdx.x; e < TS * TS; e += WG) {
+                int ly = e / TS, lx = e - ly * TS;
+                size_t c = (size_t)(tyi * TS + ly) * d.nx + txi * TS + lx;
+                dst[c] = src[c];
+            }
+        }
+    }
+}
+
+// Count non-finite values of a dense array (amps_get_field check).
+__global__ void nonfinite_kernel(const double* v, long long n, unsigned long long* cnt) {
+    unsigned long long local = 0;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        local += isfinite(v[i]) ? 0 : 1;
+    if (local) atomicAdd(cnt, local);
+}
+
+__global__ void fill_u64_kernel(unsigned long long* p, int n, unsigned long long v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void advance_step_kernel(long long* step_ctr, int n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *step_ctr += n;
+}
+
+// ============================================================ host launch
+static int hip_ok() { return hipGetLastError() == hipSuccess ? AMPS_OK : AMPS_E_HIP; }
+
+int launch_init(const InstDesc* inst_dev, int inst, long long ncells, int model, int init, int q, const Buf& b,
+                long long total_cells, hipStream_t s) {
+    int grid = (int)((ncells + 255) / 256);
+    if (grid > 65536) grid = 65536;
+    hipLaunchKernelGGL(init_kernel, dim3(grid), dim3(256), 0, s, inst_dev, inst, model, init, q, b, total_cells);
+    return hip_ok();
+}
+
+int launch_convert(const InstDesc* inst, const int* tile_inst, const Buf& b, long long total_cells, int total_tiles,
+                   const int* type_old, const int* type, const GroupMap& gm, int mode, int tile, hipStream_t s) {
+    int grid = gm.ngroups * total_tiles;
+    if (grid > 65536) grid = 65536;
+    if (tile == 16) hipLaunchKernelGGL((convert_kernel<16>), dim3(grid), dim3(256), 0, s, inst, tile_inst, b, total_cells, total_tiles, type_old, type, gm, mode);
+    else if (tile == 32) hipLaunchKernelGGL((convert_kernel<32>), dim3(grid), dim3(256), 0, s, inst, tile_inst, b, total_cells, total_tiles, type_old, type, gm, mode);
+    else if (tile == 64) hipLaunchKernelGGL((convert_kernel<64>), dim3(grid), dim3(256), 0, s, inst, tile_inst, b, total_cells, total_tiles, type_old, type, gm, mode);
+    else return AMPS_E_UNSUPPORTED;
+    return hip_ok();
+}
+
+int launch_monitor(const StepArgs& a, int kind, int tile, int ngroups_eval, int field0, int field_stride,
+                   int type_groups, double* md, double* last, hipStream_t s) {
+    const int n = ngroups_eval * a.total_tiles;
+    if (kind == AMPS_MON_GRADIENT) {
+        int grid = n > 65536 ? 65536 : n;
+        if (tile == 16) hipLaunchKernelGGL((monitor_grad_kernel<16>), dim3(grid), dim3(256), 0, s, a, ngroups_eval, field0, field_stride, type_groups, md);
+        else if (tile == 32) hipLaunchKernelGGL((monitor_grad_kernel<32>), dim3(grid), dim3(256), 0, s, a, ngroups_eval, field0, field_stride, type_groups, md);
+        else if (tile == 64) hipLaunchKernelGGL((monitor_grad_kernel<64>), dim3(grid), dim3(256), 0, s, a, ngroups_eval, field0, field_stride, type_groups, md);
+        else return AMPS_E_UNSUPPORTED;
+    } else {
+        hipLaunchKernelGGL(monitor_samples_kernel, dim3((n + 255) / 256), dim3(256), 0, s, a.inst, a.tile_inst, a.in,
+                           a.total_cells, a.total_tiles, a.type, ngroups_eval, field0, field_stride, type_groups, tile,
+                           kind, md, last);
+    }
+    return hip_ok();
+}
+
+int launch_monitor_last_init(const StepArgs& a, int tile, int ngroups_eval, int field0, int field_stride,
+                             double* last, hipStream_t s) {
+    const int n = ngroups_eval * a.total_tiles;
+    hipLaunchKernelGGL(monitor_last_init_kernel, dim3((n + 255) / 256), dim3(256), 0, s, a.inst, a.tile_inst, a.in,
+                       a.total_cells, a.total_tiles, ngroups_eval, field0, field_stride, tile, last);
+    return hip_ok();
+}
+
+int launch_classify(const InstDesc* inst, const int* tile_inst, int total_tiles, int ngroups, int broadcast,
+                    const double* md, double thr, int* type_old, int* type, unsigned long long* stats, hipStream_t s) {
+    const int n = ngroups * total_tiles;
+    hipLaunchKernelGGL(classify_kernel, dim3((n + 255) / 256), dim3(256), 0, s, inst, tile_inst, total_tiles, ngroups,
+                       broadcast, md, thr, type_old, type, stats);
+    return hip_ok();
+}
+
+int launch_gather(const InstDesc* inst, const Buf& b, long long total_cells, int total_tiles, const int* type,
+                  int group, int instid, int ntiles_inst, int field, double* dst, int tile, hipStream_t s) {
+    int grid = ntiles_inst > 65536 ? 65536 : ntiles_inst;
+    if (tile == 16) hipLaunchKernelGGL((gather_kernel<16>), dim3(grid), dim3(256), 0, s, inst, b, total_cells, total_tiles, type, group, instid, field, dst);
+    else if (tile == 32) hipLaunchKernelGGL((gather_kernel<32>), dim3(grid), dim3(256), 0, s, inst, b, total_cells, total_tiles, type, group, instid, field, dst);
+    else if (tile == 64) hipLaunchKernelGGL((gather_kernel<64>), dim3(grid), dim3(256), 0, s, inst, b, total_cells, total_tiles, type, group, instid, field, dst);
+    else return AMPS_E_UNSUPPORTED;
+    return hip_ok();
+}
+
+int launch_worklist(const int* type, int total_tiles, int* wl_hi, int* wl_lo, int* counts, hipStream_t s) {
+    hipLaunchKernelGGL(worklist_kernel, dim3(1), dim3(1024), 0, s, type, total_tiles, wl_hi, wl_lo, counts);
+    return hip_ok();
+}
+
+int launch_nonfinite(const double* v, long long n, unsigned long long* cnt, hipStream_t s) {
+    long long g = (n + 255) / 256;
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(nonfinite_kernel, dim3((int)g), dim3(256), 0, s, v, n, cnt);
+    return hip_ok();
+}
+
+int launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t s) {
+    hipLaunchKernelGGL(fill_u64_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
+    return hip_ok();
+}
+
+int launch_advance_step(long long* step_ctr, int n, hipStream_t s) {
+    hipLaunchKernelGGL(advance_step_kernel, dim3(1), dim3(64), 0, s, step_ctr, n);
+    return hip_ok();
+}
+
+}  // namespace amps

@@ -1,0 +1,185 @@
+// k_sintering.hip — Wang solid-state sintering (c, eta_1, eta_2), one fused
+// sweep per step.
+//
+// Reference: con1_pure -> con2_pure -> phi1_pure -> phi2_pure with a device
+// synchronisation after each (sintering/function.h:51-159, main.cu:136-146):
+//   mu   = B(2c + 4 sum eta^3 - 6 sum eta^2) - 2A c^2 (1-c) + 2A c (1-c)^2 - kc/2 lap c
+//   M    = Dvol h + Dvap (1-h) + Dsurf c(1-c) + Dgb 2 eta1 eta2,  h = c^3 (10 - 15c + 6c^2)
+//   c'   = clamp01(c + dt M lap mu)
+//   eta' = clamp01(eta - dt L (B(-12 eta^2 (2-c') + 12 eta (1-c') + 12 eta sum eta^2) - ke/2 lap eta))
+// with 5-point Laplacians, mu from the old state and the eta updates using
+// the NEW c and the OLD etas.  Here a workgroup stages c with a 2-cell halo
+// and the etas with a 1-cell halo, computes mu on the tile and its ring in
+// LDS (FP64) and applies all three updates: 48 B per FP64 cell-update.
+//
+// c, mu and lap mu are always FP64 (the chemical-potential divergence is the
+// precision-critical term).  In low tiles the etas are stored in fp16 with a
+// per-tile offset and scale: 16 + 2 x 4 = 24 B per cell-update.
+#include "kernels.hpp"
+#include "tile_common.hpp"
+
+namespace amps {
+
+template <int TS>
+struct SnLds {
+    static constexpr int LWC = TS + 4, LWE = TS + 2, RW = TS + 2;
+    double c[LWC * LWC];
+    double e1[LWE * LWE], e2[LWE * LWE];
+    double mu[RW * RW];
+    NbField nbf[2];
+    double red[4];
+};
+
+__device__ inline int ring_src_sn(int r, int o, int n, int bc) {
+    if (bc == AMPS_BC_PERIODIC) return r;
+    int g = o - 1 + r;
+    if (g < 0) g = -g;
+    else if (g >= n) g = 2 * (n - 1) - g;
+    return g - o + 1;
+}
+
+__device__ inline double clamp01_ref(double v) {
+    // sintering/function.h:100-101: >= 1 -> 1, < 0 -> 0
+    if (v >= 1.0) return 1.0;
+    if (v < 0.0) return 0.0;
+    return v;
+}
+
+template <int TS, bool ADAPT>
+__global__ __launch_bounds__(256) void sn_step_kernel(StepArgs a) {
+    constexpr int LWC = TS + 4, LWE = TS + 2, RW = TS + 2, PER = TS * TS / WG;
+    __shared__ SnLds<TS> L;
+    for (int b = blockIdx.x; b < a.nwork; b += gridDim.x) {
+        const int w = a.worklist ? a.worklist[b] : xcd_remap(b, a.nwork);
+        const InstDesc& d = a.inst[a.tile_inst[w]];
+        const int lt = w - d.tile_base;
+        const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
+        const int ox = tile_x * TS, oy = tile_y * TS;
+        Nb nb;
+        nb_setup(nb, d, tile_x, tile_y, TS);
+        const int ty = ADAPT ? a.type[w] : 2;
+        BoxMap<TS, 2> mc;
+        BoxMap<TS, 1> me;
+        box_map<TS, 2>(mc, d, tile_x, tile_y);
+        box_map<TS, 1>(me, d, tile_x, tile_y);
+        const double* fc = a.in.f64 + d.cell_base;
+        const double* f1 = a.in.f64 + a.total_cells + d.cell_base;
+        const double* f2 = a.in.f64 + 2 * a.total_cells + d.cell_base;
+        if (ADAPT) {
+            nbfield_fill(&L.nbf[0], a, a.in, nb, 1, 0);
+            nbfield_fill(&L.nbf[1], a, a.in, nb, 2, 0);
+            __syncthreads();
+            box_load1_f64<TS, 2>(L.c, fc, mc);
+            box_load_mixed_f64<TS, 1>(L.e1, f1, a.in.f16 + a.total_cells + d.cell_base, &L.nbf[0], me);
+            box_load_mixed_f64<TS, 1>(L.e2, f2, a.in.f16 + 2 * a.total_cells + d.cell_base, &L.nbf[1], me);
+        } else {
+            // all three fields' loads in flight together
+            constexpr int NC = BoxMap<TS, 2>::N, NE = BoxMap<TS, 1>::N;
+            double vc[NC], v1[NE], v2[NE];
+#pragma unroll
+            for (int k = 0; k < NC; ++k) if (mc.off[k] >= 0) vc[k] = fc[mc.off[k]];
+#pragma unroll
+            for (int k = 0; k < NE; ++k) if (me.off[k] >= 0) { v1[k] = f1[me.off[k]]; v2[k] = f2[me.off[k]]; }
+#pragma unroll
+            for (int k = 0; k < NC; ++k) if (mc.off[k] >= 0) L.c[threadIdx.x + k * WG] = vc[k];
+#pragma unroll
+            for (int k = 0; k < NE; ++k) if (me.off[k] >= 0) { L.e1[threadIdx.x + k * WG] = v1[k]; L.e2[threadIdx.x + k * WG] = v2[k]; }
+        }
+        __syncthreads();
+
+        const double inv = d.k[SNK_INV_DXDY], A2 = d.k[SNK_A2], B = d.k[SNK_B], hkc = d.k[SNK_HALF_KC];
+        const bool edge = d.bc != AMPS_BC_PERIODIC && (ox == 0 || oy == 0 || ox + TS == d.nx || oy + TS == d.ny);
+        // phase 1: mu on the tile and its ring (sintering/function.h:67-72)
+        for (int e = threadIdx.x; e < RW * RW; e += WG) {
+            int ry = e / RW, rx = e - ry * RW;
+            int sy = edge ? ring_src_sn(ry, oy, d.ny, d.bc) : ry, sx = edge ? ring_src_sn(rx, ox, d.nx, d.bc) : rx;
+            int cc = (sy + 1) * LWC + sx + 1;
+            int ce = sy * LWE + sx;
+            double cv = L.c[cc];
+            double con_lap = (L.c[cc - 1] + L.c[cc + 1] + L.c[cc - LWC] + L.c[cc + LWC] - 4.0 * cv) * inv;
+            double a1 = L.e1[ce], a2 = L.e2[ce];
+            double sum2 = a1 * a1 + a2 * a2;
+            double sum3 = a1 * a1 * a1 + a2 * a2 * a2;
+            double om = 1.0 - cv;
+            double df = B * (2.0 * cv + 4.0 * sum3 - 6.0 * sum2) - A2 * (cv * cv) * om + A2 * cv * (om * om);
+            L.mu[e] = df - hkc * con_lap;
+        }
+        __syncthreads();
+
+        // phase 2: c (con2_pure), then eta1, eta2 (phi1_pure, phi2_pure)
+        const double dt = d.k[SNK_DT], dvol = d.k[SNK_DVOL], dvap = d.k[SNK_DVAP], dsur = d.k[SNK_DSURF];
+        const double dgb = d.k[SNK_DGB], dtl = d.k[SNK_DT_L], hke = d.k[SNK_HALF_KE];
+        double n1[PER], n2[PER];
+        size_t gcell[PER];
+        double* oc = a.out.f64 + d.cell_base;
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            int e = threadIdx.x + i * WG;
+            int ly = e / TS, lx = e - ly * TS;
+            gcell[i] = (size_t)(oy + ly) * d.nx + ox + lx;
+            int rc = (ly + 1) * RW + lx + 1;
+            double mu_lap = (L.mu[rc - 1] + L.mu[rc + 1] + L.mu[rc - RW] + L.mu[rc + RW] - 4.0 * L.mu[rc]) * inv;
+            double cv = L.c[(ly + 2) * LWC + lx + 2];
+            int ce = (ly + 1) * LWE + lx + 1;
+            double a1 = L.e1[ce], a2 = L.e2[ce];
+            double h = cv * cv * cv * (10.0 - 15.0 * cv + 6.0 * (cv * cv));
+            double mob = dvol * h + dvap * (1.0 - h) + dsur * cv * (1.0 - cv) + dgb * (a1 * a2 * 2.0);
+            double cn = clamp01_ref(cv + dt * mob * mu_lap);
+            oc[gcell[i]] = cn;
+            double sum2 = a1 * a1 + a2 * a2;
+            double l1 = (L.e1[ce - 1] + L.e1[ce + 1] + L.e1[ce - LWE] + L.e1[ce + LWE] - 4.0 * a1) * inv;
+            double l2 = (L.e2[ce - 1] + L.e2[ce + 1] + L.e2[ce - LWE] + L.e2[ce + LWE] - 4.0 * a2) * inv;
+            double d1 = B * (-12.0 * (a1 * a1) * (2.0 - cn) + 12.0 * a1 * (1.0 - cn) + 12.0 * a1 * sum2);
+            double d2 = B * (-12.0 * (a2 * a2) * (2.0 - cn) + 12.0 * a2 * (1.0 - cn) + 12.0 * a2 * sum2);
+            n1[i] = clamp01_ref(a1 - dtl * (d1 - hke * l1));
+            n2[i] = clamp01_ref(a2 - dtl * (d2 - hke * l2));
+        }
+        if (ty == 2) {
+            double* o1 = a.out.f64 + a.total_cells + d.cell_base;
+            double* o2 = a.out.f64 + 2 * a.total_cells + d.cell_base;
+#pragma unroll
+            for (int i = 0; i < PER; ++i) { o1[gcell[i]] = n1[i]; o2[gcell[i]] = n2[i]; }
+        } else {
+            // encode both etas of the tile (offset = mid-range, power-of-two scale)
+            for (int k = 0; k < 2; ++k) {
+                double* v = k == 0 ? n1 : n2;
+                double mn = INFINITY, mx = -INFINITY;
+#pragma unroll
+                for (int i = 0; i < PER; ++i) { mn = fmin(mn, v[i]); mx = fmax(mx, v[i]); }
+                mx = block_max_d(mx, L.red);
+                __syncthreads();
+                mn = -block_max_d(-mn, L.red);
+                double off = 0.5 * (mn + mx), hw = 0.5 * (mx - mn);
+                int ex = 0;
+                if (hw > 0.0) frexp(hw, &ex);
+                if (ex < -100) ex = -100;
+                const float invs = ldexpf(1.0f, 14 - ex);
+                __half* o = a.out.f16 + (size_t)(1 + k) * a.total_cells + d.cell_base;
+#pragma unroll
+                for (int i = 0; i < PER; ++i) o[gcell[i]] = __float2half_rn((float)(v[i] - off) * invs);
+                if (threadIdx.x == 0) {
+                    a.out.off[(size_t)(1 + k) * a.total_tiles + w] = off;
+                    a.out.scl[(size_t)(1 + k) * a.total_tiles + w] = ldexpf(1.0f, ex - 14);
+                }
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+    }
+}
+
+int launch_sn_step(const StepArgs& a, int tile, bool adaptive, int grid, hipStream_t s) {
+    dim3 g(grid), blk(WG);
+    if (tile == 16) {
+        if (adaptive) hipLaunchKernelGGL((sn_step_kernel<16, true>), g, blk, 0, s, a);
+        else hipLaunchKernelGGL((sn_step_kernel<16, false>), g, blk, 0, s, a);
+    } else if (tile == 32) {
+        if (adaptive) hipLaunchKernelGGL((sn_step_kernel<32, true>), g, blk, 0, s, a);
+        else hipLaunchKernelGGL((sn_step_kernel<32, false>), g, blk, 0, s, a);
+    } else {
+        return AMPS_E_UNSUPPORTED;
+    }
+    return hipGetLastError() == hipSuccess ? AMPS_OK : AMPS_E_HIP;
+}
+
+}  // namespace amps

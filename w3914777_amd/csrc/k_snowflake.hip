@@ -1,0 +1,654 @@
+// k_snowflake.hip — Kobayashi snowflake (phi, T) explicit-Euler step for gfx950.
+//
+// One fused sweep per step replaces the reference's kernel1/kernel2 pair and
+// its six intermediate arrays (snowflake/function.h:5-60): a workgroup stages
+// phi and T of its tile plus a 2-cell halo in LDS, evaluates the anisotropy
+// (eps, eps') once per cell of the tile and its 1-cell ring, and writes only
+// the new phi and T.  Algorithmic traffic: 2 fields x (8 B read + 8 B write)
+// = 32 B per FP64 cell-update, 8 B per fp16 cell-update.
+//
+// Anisotropy without trigonometry: for integer aniso n,
+//   cos(n theta) + i sin(n theta) = ((gx + i gy) / |g|)^n,   theta = atan2(gy, gx),
+// then rotated by n*theta0 with host-computed cos/sin.  theta = atan2(0,0) = 0
+// at |g| = 0 as in the reference.  The reference evaluates
+// eps = epsb(1 + delta cos(aniso(theta - theta0))),
+// eps' = -epsb aniso delta sin(aniso(theta - theta0)) (function.h:32-34).
+//
+// Tiles of type 2 run in FP64 from FP64 storage.  Tiles of type 1 hold fp16
+// deviations from a per-tile FP64 offset (see tile_common.hpp), compute in
+// FP32 on deviations from their own offset, and re-centre offset and scale
+// after every step from the block min/max of the new values.  Halo cells are
+// read from the storage their tile's type says is current; both candidate
+// values are fetched in the first load round (with the neighbours' type,
+// offset and scale), so a tile needs a single round of memory latency.
+//
+// Data movement: every thread owns R = TS*TS/256 horizontally adjacent cells
+// of one row; it loads them with one vector load per field (FP64: 32 B,
+// fp16: 8 B), keeps them in registers for the update and writes the results
+// with one vector store per field.  The 2-cell halo frame is loaded element
+// by element, each from the storage its tile's type says is current.  LDS
+// rows are padded so that the update's neighbour reads are 16-byte aligned.
+#include "kernels.hpp"
+#include "tile_common.hpp"
+
+namespace amps {
+
+__device__ inline double rcp_any(double x) { return rcp_nr(x); }
+__device__ inline float rcp_any(float x) { return __builtin_amdgcn_rcpf(x); }
+
+template <typename R>
+__device__ inline void sf_aniso(R gx, R gy, int n, R cth0, R sth0, R epsb, R epsb_delta, R m_epsb_n_delta,
+                                R& eps, R& epsd) {
+    R mx = fmax(fabs(gx), fabs(gy));
+    R c = R(1), s = R(0);
+    if (mx != R(0)) {
+        int e;
+        frexp(mx, &e);
+        R u = ldexp(gx, -e), v = ldexp(gy, -e);  // exact scaling, max(|u|,|v|) in [0.5, 1)
+        R wr = R(1), wi = R(0), br = u, bi = v;
+        int k = n;
+        while (true) {
+            if (k & 1) {
+                R t = wr * br - wi * bi;
+                wi = wr * bi + wi * br;
+                wr = t;
+            }
+            k >>= 1;
+            if (!k) break;
+            R t = br * br - bi * bi;
+            bi = R(2) * br * bi;
+            br = t;
+        }
+        R r2 = u * u + v * v;
+        R pw = R(1), b = r2;
+        int h = n >> 1;
+        while (h) {
+            if (h & 1) pw *= b;
+            h >>= 1;
+            if (h) b *= b;
+        }
+        if (n & 1) pw *= sqrt(r2);
+        R inv = rcp_any(pw);
+        c = wr * inv;
+        s = wi * inv;
+    }
+    R ca = c * cth0 + s * sth0;  // cos(n (theta - theta0))
+    R sa = s * cth0 - c * sth0;  // sin(n (theta - theta0))
+    eps = epsb + epsb_delta * ca;
+    epsd = m_epsb_n_delta * sa;
+}
+
+// Geometry of a snowflake tile of TS x TS cells.
+template <int TS>
+struct SfGeo {
+    static constexpr int H = 2;
+    static constexpr int LW = TS + 2 * H;          // box row stride (16-byte multiple for float and double)
+    static constexpr int R = TS * TS / WG;         // cells per thread
+    static constexpr int TPR = TS / R;             // threads per row
+    static constexpr int RW = TS + 2;              // ring box rows/cols
+    // Ring arrays: ring cell (ry, rx) at RB + ry*PW + rx - 1, so that the ring
+    // cell of tile cell (ly, x) sits at RB + (ly+1)*PW + x: 16-byte aligned for
+    // x % 4 == 0.  Rows do not overlap because RW - 1 < PW.
+    static constexpr int PW = TS + 4;
+    static constexpr int RB = 4;
+    static constexpr int RSZ = RB + RW * PW;
+    static constexpr int NH = LW * LW - TS * TS;   // halo frame elements
+    static constexpr int NHT = (NH + WG - 1) / WG;
+};
+
+template <int TS, typename Rt>
+struct SfLds {
+    using G = SfGeo<TS>;
+    Rt phi[G::LW * G::LW], T[G::LW * G::LW];
+    Rt P[G::RSZ], Q[G::RSZ], E2[G::RSZ];
+    NbField nbf[2];
+    float red[8];
+};
+
+// Box coordinates of halo-frame element h (top band, bottom band, then the
+// left and right bands row by row).
+template <int TS>
+__device__ inline void halo_coord(int h, int& ly, int& lx) {
+    constexpr int H = SfGeo<TS>::H, LW = SfGeo<TS>::LW;
+    if (h < H * LW) { ly = h / LW; lx = h - ly * LW; return; }
+    h -= H * LW;
+    if (h < H * LW) { ly = H + TS + h / LW; lx = h % LW; return; }
+    h -= H * LW;
+    ly = H + h / (2 * H);
+    const int j = h % (2 * H);
+    lx = j < H ? j : TS + j;
+}
+
+// Ring coordinate used for the intermediates of ring cell r (box-relative).
+// Periodic: the box already holds the wrapped values, so r itself.  Zero
+// flux: a ring cell outside the domain takes the intermediates of its mirror
+// image inside the tile, as the oracle's index map does.
+__device__ inline int ring_src(int r, int o, int n, int bc) {
+    if (bc == AMPS_BC_PERIODIC) return r;
+    int g = o - 1 + r;
+    if (g < 0) g = -g;
+    else if (g >= n) g = 2 * (n - 1) - g;
+    return g - o + 1;
+}
+
+// Phase 1 on a box of R-typed values: P = eps eps' gx, Q = eps eps' gy, E2 = eps^2
+// over the (TS+2)^2 ring box (stored as described at SfGeo::PW).
+template <int TS, typename Rt>
+__device__ inline void sf_phase1(const Rt* phi, Rt* P, Rt* Q, Rt* E2, const InstDesc& d, int ox, int oy,
+                                 Rt inv2dx, Rt inv2dy, Rt cth0, Rt sth0, Rt epsb, Rt epsbd, Rt mend) {
+    using G = SfGeo<TS>;
+    const int n = (int)d.k[SFK_ANISO_N];
+    const bool edge = d.bc != AMPS_BC_PERIODIC &&
+                      (ox == 0 || oy == 0 || ox + TS == d.nx || oy + TS == d.ny);
+    for (int e = threadIdx.x; e < G::RW * G::RW; e += WG) {
+        const int ry0 = e / G::RW, rx0 = e - ry0 * G::RW;
+        int ry = ry0, rx = rx0;
+        if (edge) {
+            ry = ring_src(ry, oy, d.ny, d.bc);
+            rx = ring_src(rx, ox, d.nx, d.bc);
+        }
+        const int c = (ry + 1) * G::LW + rx + 1;
+        Rt gx = (phi[c + 1] - phi[c - 1]) * inv2dx;
+        Rt gy = (phi[c + G::LW] - phi[c - G::LW]) * inv2dy;
+        Rt eps, epsd;
+        sf_aniso<Rt>(gx, gy, n, cth0, sth0, epsb, epsbd, mend, eps, epsd);
+        Rt A = eps * epsd;
+        const int o = G::RB + ry0 * G::PW + rx0 - 1;
+        P[o] = A * gx;
+        Q[o] = A * gy;
+        E2[o] = eps * eps;
+    }
+}
+
+// Laplacian of the R cells (row ly, columns x0..x0+R-1) of a box field;
+// also returns the cells' own values.  Reads are 16-byte aligned rows.
+template <int TS, typename Rt>
+__device__ inline void sf_lap(const Rt* F, int ly, int x0, bool ref, Rt inv_dxdy, Rt* own, Rt* lap) {
+    using Gm = SfGeo<TS>;
+    constexpr int H = Gm::H, LW = Gm::LW, R = Gm::R;
+    const int b = (ly + H) * LW + x0;
+    Rt row[R + 4];
+#pragma unroll
+    for (int j = 0; j < R + 4; ++j) row[j] = F[b + j];
+#pragma unroll
+    for (int i = 0; i < R; ++i) own[i] = row[i + 2];
+    if (ref) {
+        Rt u1[R], u2[R], d1[R], d2[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            u1[j] = F[b - LW + H + j]; u2[j] = F[b - 2 * LW + H + j];
+            d1[j] = F[b + LW + H + j]; d2[j] = F[b + 2 * LW + H + j];
+        }
+        // order of snowflake/function.h:27-28: xs1, xs2, xa1, xa2, ys1, ys2, ya1, ya2
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+            lap[i] = (row[i + 1] + row[i] + row[i + 3] + row[i + 4] + u1[i] + u2[i] + d1[i] + d2[i] - Rt(8) * row[i + 2]) *
+                     inv_dxdy;
+    } else {
+        Rt u1[R], d1[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) { u1[j] = F[b - LW + H + j]; d1[j] = F[b + LW + H + j]; }
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+            lap[i] = (row[i + 1] + row[i + 3] + u1[i] + d1[i] - Rt(4) * row[i + 2]) * inv_dxdy;
+    }
+}
+
+// Phase 2 for the R cells (row ly, columns x0 .. x0+R-1) of this thread:
+// writes the new values to np/nt (and the old ones to pc/tc).  The noise
+// term a*phi(1-phi)(u-1/2) is evaluated only where |phi(1-phi)| > noise_skip:
+// 0 on the FP64 path (bitwise the same result), a bound far below the fp16
+// resolution on the low path.
+template <int TS, typename Rt>
+__device__ inline void sf_phase2(const Rt* F, const Rt* G, const Rt* P, const Rt* Q, const Rt* E2, int ly, int x0,
+                                 Rt* pc, Rt* tc, Rt* np, Rt* nt, bool ref, Rt inv_dxdy, Rt inv2dx,
+                                 Rt inv2dy, Rt dtt, Rt dt, Rt kap, Rt aop, Rt gam, Rt teq_minus_off, Rt phi_off,
+                                 Rt noise, Rt noise_skip, uint32_t key, int gcell0) {
+    using Gm = SfGeo<TS>;
+    constexpr int R = Gm::R, PW = Gm::PW, RB = Gm::RB;
+    Rt lp[R], lt[R];
+    sf_lap<TS, Rt>(F, ly, x0, ref, inv_dxdy, pc, lp);
+    sf_lap<TS, Rt>(G, ly, x0, ref, inv_dxdy, tc, lt);
+    // ring arrays: cell (ly, x) is ring (ly+1, x+1), stored at RB + (ly+1)*PW + x
+    const int rb = RB + (ly + 1) * PW + x0;
+    Rt flux[R];
+    {
+        Rt pu[R], pd[R], q[R + 2], e2[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) { pu[j] = P[rb - PW + j]; pd[j] = P[rb + PW + j]; e2[j] = E2[rb + j]; }
+#pragma unroll
+        for (int j = 0; j < R + 2; ++j) q[j] = Q[rb - 1 + j];
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const Rt term1 = (pd[i] - pu[i]) * inv2dy;
+            const Rt term2 = -(q[i + 2] - q[i]) * inv2dx;
+            flux[i] = term1 + term2 + e2[i] * lp[i];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        const Rt ph = pc[i], t = tc[i];
+        const Rt pa = phi_off + ph;
+        const Rt m = aop * atan(gam * (teq_minus_off - t));
+        const Rt pp = pa * (Rt(1) - pa);
+        Rt br = flux[i] + pp * (pa - Rt(0.5) + m);
+        if (noise != Rt(0) && fabs(pp) > noise_skip) {
+            const Rt u = (Rt)noise_u24(key, (uint32_t)(gcell0 + i));
+            br = br + noise * pa * (Rt(1) - pa) * (u - Rt(0.5));
+        }
+        const Rt phn = ph + dtt * br;
+        np[i] = phn;
+        nt[i] = t + dt * lt[i] + kap * (phn - ph);
+    }
+}
+
+// ------------------------------------------------------------- FP64 tile
+// Halo frame of the tile (box coordinates and cell offsets) staged by this
+// thread: element h = threadIdx.x + k*WG of the frame.
+template <int TS>
+struct SfHalo {
+    int off[SfGeo<TS>::NHT];
+    int box[SfGeo<TS>::NHT];
+    int reg[SfGeo<TS>::NHT];
+};
+
+template <int TS>
+__device__ inline void sf_halo_map(SfHalo<TS>& hm, const InstDesc& d, int ox, int oy) {
+    using G = SfGeo<TS>;
+    constexpr int H = G::H;
+    const bool interior = ox >= H && oy >= H && ox + TS + H <= d.nx && oy + TS + H <= d.ny;
+#pragma unroll
+    for (int k = 0; k < G::NHT; ++k) {
+        const int h = threadIdx.x + k * WG;
+        int bly = 0, blx = 0;
+        if (h < G::NH) halo_coord<TS>(h, bly, blx);
+        int gy = oy - H + bly, gx = ox - H + blx;
+        if (!interior) { gy = bmap(gy, d.ny, d.bc); gx = bmap(gx, d.nx, d.bc); }
+        hm.off[k] = h < G::NH ? gy * d.nx + gx : -1;
+        hm.box[k] = bly * G::LW + blx;
+        hm.reg[k] = box_region<TS, H>(bly, blx);
+    }
+}
+
+template <int TS, bool ADAPT>
+__device__ void sf_tile_f64(SfLds<TS, double>& L, const StepArgs& a, const InstDesc& d, int tile_x, int tile_y,
+                            long long step) {
+    using G = SfGeo<TS>;
+    constexpr int H = G::H, LW = G::LW, R = G::R;
+    const int ox = tile_x * TS, oy = tile_y * TS;
+    const int ly = threadIdx.x / G::TPR, x0 = (threadIdx.x % G::TPR) * R;
+    const int gcell0 = (oy + ly) * d.nx + ox + x0;
+    const double* fphi = a.in.f64 + d.cell_base;
+    const double* fT = a.in.f64 + a.total_cells + d.cell_base;
+    // own cells (vector loads) and halo elements, all loads in flight together
+    double pc[R], tc[R];
+    if (R % 2 == 0) {
+#pragma unroll
+        for (int j = 0; j < R; j += 2) {
+            const double2 u = *reinterpret_cast<const double2*>(fphi + gcell0 + j);
+            const double2 v = *reinterpret_cast<const double2*>(fT + gcell0 + j);
+            pc[j] = u.x; pc[j + 1] = u.y; tc[j] = v.x; tc[j + 1] = v.y;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < R; ++j) { pc[j] = fphi[gcell0 + j]; tc[j] = fT[gcell0 + j]; }
+    }
+    SfHalo<TS> hm;
+    sf_halo_map<TS>(hm, d, ox, oy);
+    double hp[G::NHT], ht[G::NHT];
+    __half hph[G::NHT], hth[G::NHT];
+#pragma unroll
+    for (int k = 0; k < G::NHT; ++k) {
+        if (hm.off[k] >= 0) {
+            hp[k] = fphi[hm.off[k]];
+            ht[k] = fT[hm.off[k]];
+            if (ADAPT) {
+                hph[k] = a.in.f16[d.cell_base + hm.off[k]];
+                hth[k] = a.in.f16[a.total_cells + d.cell_base + hm.off[k]];
+            }
+        }
+    }
+    if (ADAPT) {
+        Nb nb;
+        nb_setup(nb, d, tile_x, tile_y, TS);
+        nbfield_fill(&L.nbf[0], a, a.in, nb, 0, 0);
+        nbfield_fill(&L.nbf[1], a, a.in, nb, 1, 0);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        L.phi[(ly + H) * LW + x0 + H + j] = pc[j];
+        L.T[(ly + H) * LW + x0 + H + j] = tc[j];
+    }
+#pragma unroll
+    for (int k = 0; k < G::NHT; ++k) {
+        if (hm.off[k] < 0) continue;
+        const int r = hm.reg[k];
+        if (!ADAPT || L.nbf[0].type[r] == 2) {
+            L.phi[hm.box[k]] = hp[k];
+            L.T[hm.box[k]] = ht[k];
+        } else {
+            L.phi[hm.box[k]] = L.nbf[0].off[r] + (double)__half2float(hph[k]) * (double)L.nbf[0].scl[r];
+            L.T[hm.box[k]] = L.nbf[1].off[r] + (double)__half2float(hth[k]) * (double)L.nbf[1].scl[r];
+        }
+    }
+    __syncthreads();
+
+    sf_phase1<TS, double>(L.phi, L.P, L.Q, L.E2, d, ox, oy, d.k[SFK_INV_2DX], d.k[SFK_INV_2DY],
+                          d.k[SFK_COS_N_THETA0], d.k[SFK_SIN_N_THETA0], d.k[SFK_EPSB], d.k[SFK_EPSB_DELTA],
+                          d.k[SFK_MEPSB_ANISO_DELTA]);
+    __syncthreads();
+
+    double np[R], nt[R];
+    sf_phase2<TS, double>(L.phi, L.T, L.P, L.Q, L.E2, ly, x0, pc, tc, np, nt,
+                          d.stencil == AMPS_STENCIL_REF, d.k[SFK_INV_DXDY], d.k[SFK_INV_2DX], d.k[SFK_INV_2DY],
+                          d.k[SFK_DT_OVER_TAU], d.k[SFK_DT], d.k[SFK_KAPPA], d.k[SFK_ALPHA_OVER_PI], d.k[SFK_GAMMA],
+                          d.k[SFK_TEQ], 0.0, d.k[SFK_NOISE], 0.0, noise_key(d.seed, (uint64_t)step), gcell0);
+    double* ophi = a.out.f64 + d.cell_base;
+    double* oT = a.out.f64 + a.total_cells + d.cell_base;
+    if (R % 2 == 0) {
+#pragma unroll
+        for (int j = 0; j < R; j += 2) {
+            *reinterpret_cast<double2*>(ophi + gcell0 + j) = make_double2(np[j], np[j + 1]);
+            *reinterpret_cast<double2*>(oT + gcell0 + j) = make_double2(nt[j], nt[j + 1]);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < R; ++j) { ophi[gcell0 + j] = np[j]; oT[gcell0 + j] = nt[j]; }
+    }
+}
+
+// ------------------------------------------------------------- low tile
+// Bulk-phase hold: when every phi value of a low tile's box lies within
+// SF_BULK_EPS of the same stable bulk value (0 = liquid or 1 = solid; both
+// are stable fixed points of phi(1-phi)(phi-1/2+m) since |m| < alpha/2 < 1/2),
+// phi is held and only T advances (T' = T + dt lap T): the omitted phi change
+// is below ~SF_BULK_EPS in total.  The noise term is skipped where
+// |phi(1-phi)| <= SF_NOISE_SKIP_LOW (its per-step contribution is then below
+// 2e-9).
+constexpr float SF_BULK_EPS = 1.0e-8f;
+constexpr float SF_NOISE_SKIP_LOW = 1.0e-6f;
+
+// Unpack R (= 4) fp16 words of an 8-byte load.
+__device__ inline float half_of(const uint2 w, int j) {
+    const unsigned v = j < 2 ? w.x : w.y;
+    return __half2float(__ushort_as_half((unsigned short)((j & 1) ? (v >> 16) : (v & 0xFFFFu))));
+}
+
+// Encode R new deviations into fp16 words.
+template <int R>
+__device__ inline void sf_encode(const float* r, const TileCode& tc, unsigned short* w) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) w[i] = __half_as_ushort(__float2half_rn((r[i] - tc.mid) * tc.inv));
+}
+
+template <int R>
+__device__ inline void sf_store_low(__half* dst16, int gcell0, const unsigned short* w) {
+    if (R == 4) {
+        *reinterpret_cast<uint2*>(dst16 + gcell0) =
+            make_uint2((unsigned)w[0] | ((unsigned)w[1] << 16), (unsigned)w[2] | ((unsigned)w[3] << 16));
+    } else {
+#pragma unroll
+        for (int i = 0; i < R; ++i) dst16[gcell0 + i] = __ushort_as_half(w[i]);
+    }
+}
+
+// Everything a low tile needs from memory, fetched in one round so that the
+// fetch of the next tile of a workgroup overlaps the current tile's compute.
+template <int TS>
+struct LowFetch {
+    using G = SfGeo<TS>;
+    unsigned short op[G::R], ot[G::R];              // own fp16 words (phi, T)
+    double hp[G::NHT], ht[G::NHT];                  // halo FP64 candidates
+    unsigned short hph[G::NHT], hth[G::NHT];        // halo fp16 candidates
+    double offp, offT;
+    float sclp, sclT;
+    int nbt;                                        // neighbour k = threadIdx.x (< 9)
+    double nbo0, nbo1;
+    float nbs0, nbs1;
+};
+
+template <int TS>
+__device__ inline void low_fetch(LowFetch<TS>& f, const StepArgs& a, const InstDesc& d, int w) {
+    using G = SfGeo<TS>;
+    constexpr int R = G::R;
+    const int lt = w - d.tile_base;
+    const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
+    const int ox = tile_x * TS, oy = tile_y * TS;
+    const int ly = threadIdx.x / G::TPR, x0 = (threadIdx.x % G::TPR) * R;
+    const int gcell0 = (oy + ly) * d.nx + ox + x0;
+    const double* fphi = a.in.f64 + d.cell_base;
+    const double* fT = a.in.f64 + a.total_cells + d.cell_base;
+    const unsigned short* hphi = reinterpret_cast<const unsigned short*>(a.in.f16 + d.cell_base);
+    const unsigned short* hT = reinterpret_cast<const unsigned short*>(a.in.f16 + a.total_cells + d.cell_base);
+    if (R == 4) {
+        const uint2 u = *reinterpret_cast<const uint2*>(hphi + gcell0);
+        const uint2 v = *reinterpret_cast<const uint2*>(hT + gcell0);
+        f.op[0] = u.x & 0xFFFFu; f.op[1 % R] = u.x >> 16; f.op[2 % R] = u.y & 0xFFFFu; f.op[3 % R] = u.y >> 16;
+        f.ot[0] = v.x & 0xFFFFu; f.ot[1 % R] = v.x >> 16; f.ot[2 % R] = v.y & 0xFFFFu; f.ot[3 % R] = v.y >> 16;
+    } else {
+#pragma unroll
+        for (int j = 0; j < R; ++j) { f.op[j] = hphi[gcell0 + j]; f.ot[j] = hT[gcell0 + j]; }
+    }
+    SfHalo<TS> hm;
+    sf_halo_map<TS>(hm, d, ox, oy);
+#pragma unroll
+    for (int k = 0; k < G::NHT; ++k) {
+        if (hm.off[k] >= 0) {
+            f.hp[k] = fphi[hm.off[k]];
+            f.ht[k] = fT[hm.off[k]];
+            f.hph[k] = hphi[hm.off[k]];
+            f.hth[k] = hT[hm.off[k]];
+        }
+    }
+    f.offp = a.in.off[w];
+    f.offT = a.in.off[a.total_tiles + w];
+    f.sclp = a.in.scl[w];
+    f.sclT = a.in.scl[a.total_tiles + w];
+    if (threadIdx.x < 9) {
+        Nb nb;
+        nb_setup(nb, d, tile_x, tile_y, TS);
+        const int g = nb_tile(nb, threadIdx.x);
+        f.nbt = a.type[g];
+        f.nbo0 = a.in.off[g];
+        f.nbo1 = a.in.off[a.total_tiles + g];
+        f.nbs0 = a.in.scl[g];
+        f.nbs1 = a.in.scl[a.total_tiles + g];
+    }
+}
+
+template <int TS>
+__device__ void sf_tile_low(SfLds<TS, float>& L, const StepArgs& a, const InstDesc& d, int gtile,
+                            const LowFetch<TS>& f, long long step) {
+    using G = SfGeo<TS>;
+    constexpr int H = G::H, LW = G::LW, R = G::R;
+    const int lt = gtile - d.tile_base;
+    const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
+    const int ox = tile_x * TS, oy = tile_y * TS;
+    const int ly = threadIdx.x / G::TPR, x0 = (threadIdx.x % G::TPR) * R;
+    const int gcell0 = (oy + ly) * d.nx + ox + x0;
+    const double offp = f.offp, offT = f.offT;
+    const float sclp = f.sclp, sclT = f.sclT;
+    if (threadIdx.x < 9) {
+        L.nbf[0].type[threadIdx.x] = f.nbt;
+        L.nbf[1].type[threadIdx.x] = f.nbt;
+        L.nbf[0].off[threadIdx.x] = f.nbo0;
+        L.nbf[1].off[threadIdx.x] = f.nbo1;
+        L.nbf[0].scl[threadIdx.x] = f.nbs0;
+        L.nbf[1].scl[threadIdx.x] = f.nbs1;
+    }
+    float pc[R], tc[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        pc[j] = __half2float(__ushort_as_half(f.op[j])) * sclp;
+        tc[j] = __half2float(__ushort_as_half(f.ot[j])) * sclT;
+    }
+    SfHalo<TS> hm;
+    sf_halo_map<TS>(hm, d, ox, oy);
+    __syncthreads();  // neighbour table visible
+    const float c0 = (float)offp, c1 = (float)(1.0 - offp);
+    bool near0 = true, near1 = true;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        L.phi[(ly + H) * LW + x0 + H + j] = pc[j];
+        L.T[(ly + H) * LW + x0 + H + j] = tc[j];
+        near0 = near0 && fabsf(c0 + pc[j]) <= SF_BULK_EPS;
+        near1 = near1 && fabsf(c1 - pc[j]) <= SF_BULK_EPS;
+    }
+#pragma unroll
+    for (int k = 0; k < G::NHT; ++k) {
+        if (hm.off[k] < 0) continue;
+        const int r = hm.reg[k];
+        float vp;
+        if (L.nbf[0].type[r] == 2) {
+            vp = (float)(f.hp[k] - offp);
+            L.T[hm.box[k]] = (float)(f.ht[k] - offT);
+        } else {
+            vp = (float)(L.nbf[0].off[r] - offp) + __half2float(__ushort_as_half(f.hph[k])) * L.nbf[0].scl[r];
+            L.T[hm.box[k]] = (float)(L.nbf[1].off[r] - offT) + __half2float(__ushort_as_half(f.hth[k])) * L.nbf[1].scl[r];
+        }
+        L.phi[hm.box[k]] = vp;
+        near0 = near0 && fabsf(c0 + vp) <= SF_BULK_EPS;
+        near1 = near1 && fabsf(c1 - vp) <= SF_BULK_EPS;
+    }
+    const bool hold = __syncthreads_and(near0) || __syncthreads_and(near1);
+    __half* dphi = a.out.f16 + d.cell_base;
+    __half* dT = a.out.f16 + a.total_cells + d.cell_base;
+    if (hold) {
+        // phi held (same fp16 words, offset and scale); T' = T + dt lap T
+        float lt[R];
+        sf_lap<TS, float>(L.T, ly, x0, d.stencil == AMPS_STENCIL_REF, d.kf[SFK_INV_DXDY], tc, lt);
+        float mnt = INFINITY, mxt = -INFINITY;
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            tc[i] = tc[i] + d.kf[SFK_DT] * lt[i];
+            mnt = fminf(mnt, tc[i]);
+            mxt = fmaxf(mxt, tc[i]);
+        }
+        block_minmax(mnt, mxt, L.red);
+        const TileCode ct = tile_code(mnt, mxt);
+        const double offTn = offT + (double)ct.mid;
+        unsigned short w[R];
+        sf_encode<R>(tc, ct, w);
+        sf_store_low<R>(dT, gcell0, w);
+        sf_store_low<R>(dphi, gcell0, f.op);
+        if (threadIdx.x == 0) {
+            a.out.off[gtile] = offp;
+            a.out.scl[gtile] = sclp;
+            a.out.off[a.total_tiles + gtile] = offTn;
+            a.out.scl[a.total_tiles + gtile] = ct.scl;
+        }
+        return;
+    }
+
+    sf_phase1<TS, float>(L.phi, L.P, L.Q, L.E2, d, ox, oy, d.kf[SFK_INV_2DX], d.kf[SFK_INV_2DY],
+                         d.kf[SFK_COS_N_THETA0], d.kf[SFK_SIN_N_THETA0], d.kf[SFK_EPSB], d.kf[SFK_EPSB_DELTA],
+                         d.kf[SFK_MEPSB_ANISO_DELTA]);
+    __syncthreads();
+
+    float rp[R], rt[R];
+    sf_phase2<TS, float>(L.phi, L.T, L.P, L.Q, L.E2, ly, x0, pc, tc, rp, rt,
+                         d.stencil == AMPS_STENCIL_REF, d.kf[SFK_INV_DXDY], d.kf[SFK_INV_2DX], d.kf[SFK_INV_2DY],
+                         d.kf[SFK_DT_OVER_TAU], d.kf[SFK_DT], d.kf[SFK_KAPPA], d.kf[SFK_ALPHA_OVER_PI],
+                         d.kf[SFK_GAMMA], (float)(d.k[SFK_TEQ] - offT), (float)offp, d.kf[SFK_NOISE],
+                         SF_NOISE_SKIP_LOW, noise_key(d.seed, (uint64_t)step), gcell0);
+    float mnp = INFINITY, mxp = -INFINITY, mnt = INFINITY, mxt = -INFINITY;
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        mnp = fminf(mnp, rp[i]); mxp = fmaxf(mxp, rp[i]);
+        mnt = fminf(mnt, rt[i]); mxt = fmaxf(mxt, rt[i]);
+    }
+    block_minmax(mnp, mxp, L.red);
+    const TileCode cp = tile_code(mnp, mxp);
+    block_minmax(mnt, mxt, L.red);
+    const TileCode ct = tile_code(mnt, mxt);
+    const double offpn = offp + (double)cp.mid, offTn = offT + (double)ct.mid;
+    unsigned short w[R];
+    sf_encode<R>(rp, cp, w);
+    sf_store_low<R>(dphi, gcell0, w);
+    sf_encode<R>(rt, ct, w);
+    sf_store_low<R>(dT, gcell0, w);
+    if (threadIdx.x == 0) {
+        a.out.off[gtile] = offpn;
+        a.out.scl[gtile] = cp.scl;
+        a.out.off[a.total_tiles + gtile] = offTn;
+        a.out.scl[a.total_tiles + gtile] = ct.scl;
+    }
+}
+
+// ---------------------------------------------------------------- kernels
+// Persistent sweeps over a tile work-list: the FP64 kernel takes every tile
+// (PURE_FP64) or the FP64 list, the low kernel the low-precision list; each
+// list is built on the device after every classification, in tile order.
+template <int TS, bool ADAPT>
+__global__ __launch_bounds__(256, 3) void sf_high_kernel(StepArgs a) {
+    __shared__ __align__(16) SfLds<TS, double> L;
+    const long long step = *a.step_ctr + a.step_add;
+    const int n = a.wl_count ? *a.wl_count : a.nwork;
+    for (int b = blockIdx.x; b < n; b += gridDim.x) {
+        const int e = xcd_remap(b, n);
+        const int w = a.worklist ? a.worklist[e] : e;
+        const InstDesc& d = a.inst[a.tile_inst[w]];
+        const int lt = w - d.tile_base;
+        const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
+        sf_tile_f64<TS, ADAPT>(L, a, d, tile_x, tile_y, step);
+        __syncthreads();
+    }
+}
+
+template <int TS>
+__global__ __launch_bounds__(256) void sf_low_kernel(StepArgs a) {
+    __shared__ __align__(16) SfLds<TS, float> L;
+    const long long step = *a.step_ctr + a.step_add;
+    const int n = *a.wl_low_count, G = gridDim.x;
+    int b = blockIdx.x;
+    if (b >= n) return;
+    // tile i is computed while tile i+1's data and tile i+2's work-list entry load
+    int w = a.wl_low[xcd_remap(b, n)];
+    const InstDesc* dp = &a.inst[a.tile_inst[w]];
+    LowFetch<TS> cur;
+    low_fetch<TS>(cur, a, *dp, w);
+    int w1 = b + G < n ? a.wl_low[xcd_remap(b + G, n)] : -1;
+    while (true) {
+        LowFetch<TS> nxt;
+        const InstDesc* dp1 = dp;
+        if (w1 >= 0) {
+            dp1 = &a.inst[a.tile_inst[w1]];
+            low_fetch<TS>(nxt, a, *dp1, w1);
+        }
+        const int w2 = b + 2 * G < n ? a.wl_low[xcd_remap(b + 2 * G, n)] : -1;
+        sf_tile_low<TS>(L, a, *dp, w, cur, step);
+        __syncthreads();
+        if (w1 < 0) break;
+        cur = nxt;
+        w = w1;
+        dp = dp1;
+        w1 = w2;
+        b += G;
+    }
+}
+
+// ------------------------------------------------------------ host launch
+int launch_sf_step(const StepArgs& a, int tile, bool adaptive, int grid_hi, int grid_lo, hipStream_t s) {
+    dim3 blk(WG);
+    if (tile == 16) {
+        if (adaptive) {
+            hipLaunchKernelGGL((sf_high_kernel<16, true>), dim3(grid_hi), blk, 0, s, a);
+            hipLaunchKernelGGL((sf_low_kernel<16>), dim3(grid_lo), blk, 0, s, a);
+        } else {
+            hipLaunchKernelGGL((sf_high_kernel<16, false>), dim3(grid_hi), blk, 0, s, a);
+        }
+    } else if (tile == 32) {
+        if (adaptive) {
+            hipLaunchKernelGGL((sf_high_kernel<32, true>), dim3(grid_hi), blk, 0, s, a);
+            hipLaunchKernelGGL((sf_low_kernel<32>), dim3(grid_lo), blk, 0, s, a);
+        } else {
+            hipLaunchKernelGGL((sf_high_kernel<32, false>), dim3(grid_hi), blk, 0, s, a);
+        }
+    } else {
+        return AMPS_E_UNSUPPORTED;
+    }
+    return hipGetLastError() == hipSuccess ? AMPS_OK : AMPS_E_HIP;
+}
+
+}  // namespace amps

@@ -1,0 +1,39 @@
+// kernels.hpp — host-side launch entry points of the gfx950 kernels.
+#pragma once
+
+#include "amps_internal.hpp"
+
+namespace amps {
+
+// Which type group each field belongs to (-1: always FP64).
+struct GroupMap {
+    int nfields;
+    int ngroups;
+    signed char fg[AMPS_MAX_FIELDS];
+};
+
+enum { CONVERT_CHANGED = 0, CONVERT_ENCODE_LOW = 1, CONVERT_DECODE_LOW = 2 };
+
+int launch_init(const InstDesc* inst_dev, int inst, long long ncells, int model, int init, int q, const Buf& b,
+                long long total_cells, hipStream_t s);
+int launch_convert(const InstDesc* inst, const int* tile_inst, const Buf& b, long long total_cells, int total_tiles,
+                   const int* type_old, const int* type, const GroupMap& gm, int mode, int tile, hipStream_t s);
+int launch_monitor(const StepArgs& a, int kind, int tile, int ngroups_eval, int field0, int field_stride,
+                   int type_groups, double* md, double* last, hipStream_t s);
+int launch_monitor_last_init(const StepArgs& a, int tile, int ngroups_eval, int field0, int field_stride,
+                             double* last, hipStream_t s);
+int launch_classify(const InstDesc* inst, const int* tile_inst, int total_tiles, int ngroups, int broadcast,
+                    const double* md, double thr, int* type_old, int* type, unsigned long long* stats, hipStream_t s);
+int launch_gather(const InstDesc* inst, const Buf& b, long long total_cells, int total_tiles, const int* type,
+                  int group, int instid, int ntiles_inst, int field, double* dst, int tile, hipStream_t s);
+int launch_nonfinite(const double* v, long long n, unsigned long long* cnt, hipStream_t s);
+int launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t s);
+int launch_advance_step(long long* step_ctr, int n, hipStream_t s);
+
+int launch_sf_step(const StepArgs& a, int tile, bool adaptive, int grid_hi, int grid_lo, hipStream_t s);
+int launch_worklist(const int* type, int total_tiles, int* wl_hi, int* wl_lo, int* counts, hipStream_t s);
+int launch_gg_step(const StepArgs& a, int tile, bool adaptive, int grid, hipStream_t s);
+int launch_gg_aux(const StepArgs& a, int tile, hipStream_t s);
+int launch_sn_step(const StepArgs& a, int tile, bool adaptive, int grid, hipStream_t s);
+
+}  // namespace amps

@@ -1,0 +1,265 @@
+// tile_common.hpp — device building blocks shared by the model kernels:
+// neighbour-tile table, precision-aware halo loads into LDS, block
+// reductions and the per-tile fp16 encoder.
+//
+// A workgroup (256 threads) owns one square tile of TS x TS cells and stages
+// the tile plus an H-cell halo ("box", LW = TS + 2H per side) in LDS.  Cells
+// of a halo may belong to neighbour tiles held in the other precision; the
+// loaders read each neighbour from the storage its type says is current:
+//   FP64 tile: f64[cell]
+//   low tile : off[tile] + f16[cell] * scl[tile]
+#pragma once
+
+#include "amps_internal.hpp"
+
+namespace amps {
+
+constexpr int WG = 256;
+
+// Type of (tile, group); group < 0 means the field is always FP64.
+__device__ inline int tile_type(const StepArgs& a, int group, int gtile) {
+    if (a.type == nullptr || group < 0) return 2;
+    return a.type[(size_t)group * a.total_tiles + gtile];
+}
+
+// The tiles around the workgroup's tile, in box order k = (dy+1)*3 + (dx+1).
+// For zero-flux edges the out-of-domain direction maps back into the own tile
+// (mirror), for periodic edges to the wrapped tile.
+struct Nb {
+    const InstDesc* d;
+    int tile_x, tile_y, TS;
+};
+
+__device__ inline void nb_setup(Nb& nb, const InstDesc& d, int tile_x, int tile_y, int TS) {
+    nb.d = &d;
+    nb.tile_x = tile_x;
+    nb.tile_y = tile_y;
+    nb.TS = TS;
+}
+
+__device__ inline int nb_tile(const Nb& nb, int k) {
+    const InstDesc& d = *nb.d;
+    const int TS = nb.TS, ox = nb.tile_x * TS, oy = nb.tile_y * TS;
+    const int dy = k / 3 - 1, dx = k % 3 - 1;
+    const int gx = bmap(dx < 0 ? ox - 1 : (dx > 0 ? ox + TS : ox), d.nx, d.bc);
+    const int gy = bmap(dy < 0 ? oy - 1 : (dy > 0 ? oy + TS : oy), d.ny, d.bc);
+    return d.tile_base + (gy / TS) * d.tx + gx / TS;
+}
+
+// Box region (0..8) of box coordinates (ly, lx).
+template <int TS, int H>
+__device__ inline int box_region(int ly, int lx) {
+    int rx = lx < H ? 0 : (lx < H + TS ? 1 : 2);
+    int ry = ly < H ? 0 : (ly < H + TS ? 1 : 2);
+    return ry * 3 + rx;
+}
+
+// Per-workgroup cache of the neighbour tiles' precision state for one field.
+struct NbField {
+    int type[9];
+    double off[9];
+    float scl[9];
+};
+
+// Threads 0..8 record the type, offset and scale of neighbour k = threadIdx.x
+// (offset and scale are read whatever the type, so no load waits on another).
+__device__ inline void nbfield_fill(NbField* s, const StepArgs& a, const Buf& b, const Nb& nb, int field,
+                                    int group) {
+    int t = threadIdx.x;
+    if (t < 9) {
+        int g = nb_tile(nb, t);
+        int ty = tile_type(a, group, g);
+        double off = b.off ? b.off[(size_t)field * a.total_tiles + g] : 0.0;
+        float scl = b.scl ? b.scl[(size_t)field * a.total_tiles + g] : 0.0f;
+        s->type[t] = ty;
+        s->off[t] = off;
+        s->scl[t] = scl;
+    }
+}
+
+// ------------------------------------------------------- batched box load
+// Offsets (inside one instance's field array) of the box elements a thread
+// stages: element e = threadIdx.x + k*WG.  Computed once per tile and shared
+// by every field of the tile; interior tiles skip the boundary map.
+template <int TS, int H>
+struct BoxMap {
+    static constexpr int LW = TS + 2 * H, NE = LW * LW, N = (NE + WG - 1) / WG;
+    int off[N];
+    int reg[N];   // 3x3 region of the element (neighbour table index)
+};
+
+template <int TS, int H>
+__device__ inline void box_map(BoxMap<TS, H>& m, const InstDesc& d, int tile_x, int tile_y) {
+    constexpr int LW = TS + 2 * H, NE = LW * LW, N = BoxMap<TS, H>::N;
+    const int ox = tile_x * TS - H, oy = tile_y * TS - H;
+    const bool interior = ox >= 0 && oy >= 0 && ox + LW <= d.nx && oy + LW <= d.ny;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const int e = threadIdx.x + k * WG;
+        const int ly = e / LW, lx = e - ly * LW;
+        int gy = oy + ly, gx = ox + lx;
+        if (!interior) {
+            gy = bmap(gy, d.ny, d.bc);
+            gx = bmap(gx, d.nx, d.bc);
+        }
+        m.off[k] = (e < NE) ? gy * d.nx + gx : -1;
+        m.reg[k] = box_region<TS, H>(ly, lx);
+    }
+}
+
+// Stage two FP64 fields whose every tile is FP64.  All loads are issued
+// before the first LDS store so their latencies overlap.
+template <int TS, int H>
+__device__ inline void box_load2_f64(double* b0, double* b1, const double* f0, const double* f1,
+                                     const BoxMap<TS, H>& m) {
+    constexpr int N = BoxMap<TS, H>::N;
+    double v0[N], v1[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        if (m.off[k] >= 0) {
+            v0[k] = f0[m.off[k]];
+            v1[k] = f1[m.off[k]];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        if (m.off[k] >= 0) {
+            b0[threadIdx.x + k * WG] = v0[k];
+            b1[threadIdx.x + k * WG] = v1[k];
+        }
+    }
+}
+
+template <int TS, int H>
+__device__ inline void box_load1_f64(double* b0, const double* f0, const BoxMap<TS, H>& m) {
+    constexpr int N = BoxMap<TS, H>::N;
+    double v0[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+        if (m.off[k] >= 0) v0[k] = f0[m.off[k]];
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+        if (m.off[k] >= 0) b0[threadIdx.x + k * WG] = v0[k];
+}
+
+// Stage one field as FP64 values when neighbour tiles may be low precision
+// (decoded exactly: off + h * scl).
+template <int TS, int H>
+__device__ inline void box_load_mixed_f64(double* b0, const double* f64, const __half* f16, const NbField* nbf,
+                                          const BoxMap<TS, H>& m) {
+    constexpr int N = BoxMap<TS, H>::N;
+    double v[N];
+    __half h[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        if (m.off[k] >= 0) {
+            if (nbf->type[m.reg[k]] == 2) v[k] = f64[m.off[k]];
+            else h[k] = f16[m.off[k]];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        if (m.off[k] >= 0) {
+            const int r = m.reg[k];
+            b0[threadIdx.x + k * WG] = nbf->type[r] == 2 ? v[k] : nbf->off[r] + (double)__half2float(h[k]) * (double)nbf->scl[r];
+        }
+    }
+}
+
+// Stage one field as FP32 deviations from own_off (the workgroup's tile is low).
+template <int TS, int H>
+__device__ inline void box_load_dev32(float* b0, const double* f64, const __half* f16, const NbField* nbf,
+                                      double own_off, const BoxMap<TS, H>& m) {
+    constexpr int N = BoxMap<TS, H>::N;
+    double v[N];
+    __half h[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        if (m.off[k] >= 0) {
+            if (nbf->type[m.reg[k]] == 2) v[k] = f64[m.off[k]];
+            else h[k] = f16[m.off[k]];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        if (m.off[k] >= 0) {
+            const int r = m.reg[k];
+            float x;
+            if (nbf->type[r] == 2) x = (float)(v[k] - own_off);
+            else if (r == 4) x = __half2float(h[k]) * nbf->scl[4];
+            else x = (float)(nbf->off[r] - own_off) + __half2float(h[k]) * nbf->scl[r];
+            b0[threadIdx.x + k * WG] = x;
+        }
+    }
+}
+
+// 1/x to ~1 ulp: hardware reciprocal estimate refined by two Newton steps.
+__device__ inline double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return r;
+}
+
+// ----------------------------------------------------------- reductions
+__device__ inline float wave_min(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline double wave_max_d(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Min and max over the workgroup; red must hold 2*WG/64 floats.  Contains
+// barriers: every thread must call it.
+__device__ inline void block_minmax(float& mn, float& mx, float* red) {
+    mn = wave_min(mn);
+    mx = wave_max(mx);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) { red[w] = mn; red[4 + w] = mx; }
+    __syncthreads();
+    mn = fminf(fminf(red[0], red[1]), fminf(red[2], red[3]));
+    mx = fmaxf(fmaxf(red[4], red[5]), fmaxf(red[6], red[7]));
+}
+
+__device__ inline double block_max_d(double v, double* red) {
+    v = wave_max_d(v);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) red[w] = v;
+    __syncthreads();
+    return fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+// ------------------------------------------------------------ fp16 codec
+// A low tile stores h = (value - off) / scl in fp16 (round to nearest even),
+// where off is the tile's mid-range and scl = 2^(e-14) for a half-range
+// below 2^e, so |h| < 2^14 (fp16 max 65504) and the scaling is exact.
+struct TileCode {
+    float mid;   // added to the old offset
+    float inv;   // 1/scl
+    float scl;
+};
+
+__device__ inline TileCode tile_code(float mn, float mx) {
+    TileCode t;
+    t.mid = 0.5f * (mn + mx);
+    float hw = 0.5f * (mx - mn);
+    int ex = 0;
+    if (hw > 0.0f && hw <= 3.0e38f) frexpf(hw, &ex);
+    if (ex < -100) ex = -100;
+    t.scl = ldexpf(1.0f, ex - 14);
+    t.inv = ldexpf(1.0f, 14 - ex);
+    return t;
+}
+
+}  // namespace amps

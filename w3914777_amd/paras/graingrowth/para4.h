@@ -1,0 +1,4 @@
+// Grain-growth parameter set 4 (values of the reference's graingrowth/paras/para4.h).
+const highprecision dtime=0.005,mobil=5.0,dx=0.5,dy=0.5,dxdy=dx*dy,grcoef=0.1;
+const float R=80;
+const float Ry=dimY/2,Rx=dimX/2;
