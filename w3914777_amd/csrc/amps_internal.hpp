@@ -119,7 +119,7 @@ struct StepArgs {
     int q;                      // grain growth order parameters
     const long long* step_ctr;  // device step counter (steps before this launch)
     int step_add;               // added to *step_ctr
-    int pad;
+    int ninst;                  // instances in the context (1: descriptor is loop-invariant)
     Buf in, out;
     double* aux_in;             // model auxiliary per-cell array (grain growth: sum of squares)
     double* aux_out;

@@ -80,7 +80,6 @@ struct amps_ctx {
     double* d_scratch = nullptr;           // max instance cells (get/set)
     int* d_wl = nullptr;                   // [2][total_tiles]: FP64 and low tile work-lists
     int* d_wl_count = nullptr;             // [2]
-    int grid_hi = 0, grid_lo = 0;          // persistent grid sizes of the snowflake sweeps
 
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -242,6 +241,7 @@ StepArgs step_args(amps_ctx* c, int from) {
     a.q = c->q;
     a.step_ctr = c->d_step;
     a.step_add = 0;
+    a.ninst = c->ninst;
     a.in = c->buf[from];
     a.out = c->buf[from ^ 1];
     a.aux_in = c->d_aux[from];
@@ -266,8 +266,7 @@ int launch_step_kernel(amps_ctx* c, const StepArgs& a) {
     const bool adaptive = has_low(c);
     const int grid = a.nwork;
     if (c->model == AMPS_SNOWFLAKE)
-        return launch_sf_step(a, c->tile, adaptive, adaptive ? c->grid_hi : std::min(grid, c->grid_hi), c->grid_lo,
-                              c->stream);
+        return launch_sf_step(a, c->tile, adaptive, c->stream);
     if (c->model == AMPS_GRAINGROWTH) return launch_gg_step(a, c->tile, adaptive, grid, c->stream);
     return launch_sn_step(a, c->tile, adaptive, grid, c->stream);
 }
@@ -548,15 +547,6 @@ int amps_create_batch(const amps_config* cfgs, int n, amps_ctx** out) {
     e = hipMemcpy(c->d_desc, c->desc.data(), sizeof(InstDesc) * n, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(c->d_tile_inst, c->tile_inst.data(), sizeof(int) * tiles, hipMemcpyHostToDevice);
     if (e != hipSuccess) { amps_destroy(c); return fail(AMPS_E_HIP, std::string("upload: ") + hipGetErrorString(e)); }
-    int cus = 256;
-    int dev = 0;
-    if (hipGetDevice(&dev) == hipSuccess) {
-        int v = 0;
-        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) cus = v;
-    }
-    // persistent grids: resident workgroups per CU of each sweep kernel
-    c->grid_hi = std::min(tiles, cus * 3);
-    c->grid_lo = std::min(tiles, cus * 5);
     const char* ng = std::getenv("AMPS_NO_GRAPH");
     c->use_graph = !(ng && ng[0] == '1');
     c->graph_period = c->mode == AMPS_ADAPTIVE ? (c->every % 2 ? 2 * c->every : c->every) : 10;

@@ -41,13 +41,17 @@ __device__ inline T gg_lap(const T* B, int c, int LW, const T* cv, int i, bool m
     return (B[c - 1] + B[c + 1] + cv[i] + cv[i + 2] - T(4) * cv[i + 1]) * inv;
 }
 
-template <int TS, bool ADAPT>
-__global__ __launch_bounds__(256) void gg_step_kernel(StepArgs a) {
+// Read-only metadata (descriptors, tile -> instance map, tile types) arrive
+// as restrict-qualified arguments so that the per-tile reads go through the
+// scalar cache; with MULTI = false the descriptor is loop-invariant.
+template <int TS, bool ADAPT, bool MULTI>
+__global__ __launch_bounds__(256) void gg_step_kernel(StepArgs a, const InstDesc* __restrict__ inst,
+                                                      const int* __restrict__ tinst, const int* __restrict__ types) {
     constexpr int H = 1, LW = TS + 2, R = TS * TS / WG;
     __shared__ GgLds<TS> L;
     for (int b = blockIdx.x; b < a.nwork; b += gridDim.x) {
-        const int w = a.worklist ? a.worklist[b] : xcd_remap(b, a.nwork);
-        const InstDesc& d = a.inst[a.tile_inst[w]];
+        const int w = xcd_remap(b, a.nwork);
+        const InstDesc& d = inst[MULTI ? tinst[w] : 0];
         const int lt = w - d.tile_base;
         const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
         const int ox = tile_x * TS, oy = tile_y * TS;
@@ -69,7 +73,7 @@ __global__ __launch_bounds__(256) void gg_step_kernel(StepArgs a) {
             Sn[i] = 0.0;
         }
         for (int f = 0; f < a.q; ++f) {
-            const int ty = ADAPT ? a.type[(size_t)f * a.total_tiles + w] : 2;
+            const int ty = ADAPT ? types[(size_t)f * a.total_tiles + w] : 2;
             const double* f64 = a.in.f64 + (size_t)f * a.total_cells + d.cell_base;
             if (ADAPT) nbfield_fill(&L.nbf, a, a.in, nb, f, f);
             __syncthreads();
@@ -179,14 +183,21 @@ __global__ __launch_bounds__(256) void gg_aux_kernel(StepArgs a) {
     }
 }
 
+template <int TS, bool ADAPT>
+static void gg_launch(const StepArgs& a, int grid, hipStream_t s) {
+    if (a.ninst > 1)
+        hipLaunchKernelGGL((gg_step_kernel<TS, ADAPT, true>), dim3(grid), dim3(WG), 0, s, a, a.inst, a.tile_inst, a.type);
+    else
+        hipLaunchKernelGGL((gg_step_kernel<TS, ADAPT, false>), dim3(grid), dim3(WG), 0, s, a, a.inst, a.tile_inst, a.type);
+}
+
 int launch_gg_step(const StepArgs& a, int tile, bool adaptive, int grid, hipStream_t s) {
-    dim3 g(grid), blk(WG);
     if (tile == 16) {
-        if (adaptive) hipLaunchKernelGGL((gg_step_kernel<16, true>), g, blk, 0, s, a);
-        else hipLaunchKernelGGL((gg_step_kernel<16, false>), g, blk, 0, s, a);
+        if (adaptive) gg_launch<16, true>(a, grid, s);
+        else gg_launch<16, false>(a, grid, s);
     } else if (tile == 32) {
-        if (adaptive) hipLaunchKernelGGL((gg_step_kernel<32, true>), g, blk, 0, s, a);
-        else hipLaunchKernelGGL((gg_step_kernel<32, false>), g, blk, 0, s, a);
+        if (adaptive) gg_launch<32, true>(a, grid, s);
+        else gg_launch<32, false>(a, grid, s);
     } else {
         return AMPS_E_UNSUPPORTED;
     }

@@ -45,19 +45,23 @@ __device__ inline double clamp01_ref(double v) {
     return v;
 }
 
-template <int TS, bool ADAPT>
-__global__ __launch_bounds__(256) void sn_step_kernel(StepArgs a) {
+// Read-only metadata (descriptors, tile -> instance map, tile types) arrive
+// as restrict-qualified arguments so that the per-tile reads go through the
+// scalar cache; with MULTI = false the descriptor is loop-invariant.
+template <int TS, bool ADAPT, bool MULTI>
+__global__ __launch_bounds__(256) void sn_step_kernel(StepArgs a, const InstDesc* __restrict__ inst,
+                                                      const int* __restrict__ tinst, const int* __restrict__ types) {
     constexpr int LWC = TS + 4, LWE = TS + 2, RW = TS + 2, PER = TS * TS / WG;
     __shared__ SnLds<TS> L;
     for (int b = blockIdx.x; b < a.nwork; b += gridDim.x) {
-        const int w = a.worklist ? a.worklist[b] : xcd_remap(b, a.nwork);
-        const InstDesc& d = a.inst[a.tile_inst[w]];
+        const int w = xcd_remap(b, a.nwork);
+        const InstDesc& d = inst[MULTI ? tinst[w] : 0];
         const int lt = w - d.tile_base;
         const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
         const int ox = tile_x * TS, oy = tile_y * TS;
         Nb nb;
         nb_setup(nb, d, tile_x, tile_y, TS);
-        const int ty = ADAPT ? a.type[w] : 2;
+        const int ty = ADAPT ? types[w] : 2;
         BoxMap<TS, 2> mc;
         BoxMap<TS, 1> me;
         box_map<TS, 2>(mc, d, tile_x, tile_y);
@@ -168,14 +172,21 @@ __global__ __launch_bounds__(256) void sn_step_kernel(StepArgs a) {
     }
 }
 
+template <int TS, bool ADAPT>
+static void sn_launch(const StepArgs& a, int grid, hipStream_t s) {
+    if (a.ninst > 1)
+        hipLaunchKernelGGL((sn_step_kernel<TS, ADAPT, true>), dim3(grid), dim3(WG), 0, s, a, a.inst, a.tile_inst, a.type);
+    else
+        hipLaunchKernelGGL((sn_step_kernel<TS, ADAPT, false>), dim3(grid), dim3(WG), 0, s, a, a.inst, a.tile_inst, a.type);
+}
+
 int launch_sn_step(const StepArgs& a, int tile, bool adaptive, int grid, hipStream_t s) {
-    dim3 g(grid), blk(WG);
     if (tile == 16) {
-        if (adaptive) hipLaunchKernelGGL((sn_step_kernel<16, true>), g, blk, 0, s, a);
-        else hipLaunchKernelGGL((sn_step_kernel<16, false>), g, blk, 0, s, a);
+        if (adaptive) sn_launch<16, true>(a, grid, s);
+        else sn_launch<16, false>(a, grid, s);
     } else if (tile == 32) {
-        if (adaptive) hipLaunchKernelGGL((sn_step_kernel<32, true>), g, blk, 0, s, a);
-        else hipLaunchKernelGGL((sn_step_kernel<32, false>), g, blk, 0, s, a);
+        if (adaptive) sn_launch<32, true>(a, grid, s);
+        else sn_launch<32, false>(a, grid, s);
     } else {
         return AMPS_E_UNSUPPORTED;
     }

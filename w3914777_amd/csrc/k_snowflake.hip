@@ -580,71 +580,97 @@ __device__ void sf_tile_low(SfLds<TS, float>& L, const StepArgs& a, const InstDe
 // Persistent sweeps over a tile work-list: the FP64 kernel takes every tile
 // (PURE_FP64) or the FP64 list, the low kernel the low-precision list; each
 // list is built on the device after every classification, in tile order.
-template <int TS, bool ADAPT>
-__global__ __launch_bounds__(256, 3) void sf_high_kernel(StepArgs a) {
+//
+// The read-only metadata (instance descriptors, tile -> instance map, the
+// work-list and its count) arrive as separate restrict-qualified arguments:
+// the compiler then reads them through the scalar cache instead of a chain of
+// vector loads and waits per tile, and with MULTI = false (one instance) the
+// descriptor is loaded once per workgroup.  The next work-list entry is read
+// one tile ahead.
+template <int TS, bool ADAPT, bool MULTI>
+__global__ __launch_bounds__(256, 3) void sf_high_kernel(StepArgs a, const InstDesc* __restrict__ inst,
+                                                         const int* __restrict__ tinst, const int* __restrict__ wl,
+                                                         const int* __restrict__ wlc) {
     __shared__ __align__(16) SfLds<TS, double> L;
     const long long step = *a.step_ctr + a.step_add;
-    const int n = a.wl_count ? *a.wl_count : a.nwork;
-    for (int b = blockIdx.x; b < n; b += gridDim.x) {
-        const int e = xcd_remap(b, n);
-        const int w = a.worklist ? a.worklist[e] : e;
-        const InstDesc& d = a.inst[a.tile_inst[w]];
+    const int n = wlc ? *wlc : a.nwork, G = gridDim.x;
+    int b = blockIdx.x;
+    int w = b < n ? (wl ? wl[xcd_remap(b, n)] : xcd_remap(b, n)) : 0;
+    for (; b < n; b += G) {
+        const int wn = b + G < n ? (wl ? wl[xcd_remap(b + G, n)] : xcd_remap(b + G, n)) : 0;
+        const InstDesc& d = inst[MULTI ? tinst[w] : 0];
         const int lt = w - d.tile_base;
         const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
         sf_tile_f64<TS, ADAPT>(L, a, d, tile_x, tile_y, step);
         __syncthreads();
+        w = wn;
     }
 }
 
-template <int TS>
-__global__ __launch_bounds__(256) void sf_low_kernel(StepArgs a) {
+template <int TS, bool MULTI>
+__global__ __launch_bounds__(256) void sf_low_kernel(StepArgs a, const InstDesc* __restrict__ inst,
+                                                     const int* __restrict__ tinst, const int* __restrict__ wl,
+                                                     const int* __restrict__ wlc) {
     __shared__ __align__(16) SfLds<TS, float> L;
     const long long step = *a.step_ctr + a.step_add;
-    const int n = *a.wl_low_count, G = gridDim.x;
+    const int n = *wlc, G = gridDim.x;
     int b = blockIdx.x;
-    if (b >= n) return;
-    // tile i is computed while tile i+1's data and tile i+2's work-list entry load
-    int w = a.wl_low[xcd_remap(b, n)];
-    const InstDesc* dp = &a.inst[a.tile_inst[w]];
-    LowFetch<TS> cur;
-    low_fetch<TS>(cur, a, *dp, w);
-    int w1 = b + G < n ? a.wl_low[xcd_remap(b + G, n)] : -1;
-    while (true) {
-        LowFetch<TS> nxt;
-        const InstDesc* dp1 = dp;
-        if (w1 >= 0) {
-            dp1 = &a.inst[a.tile_inst[w1]];
-            low_fetch<TS>(nxt, a, *dp1, w1);
-        }
-        const int w2 = b + 2 * G < n ? a.wl_low[xcd_remap(b + 2 * G, n)] : -1;
-        sf_tile_low<TS>(L, a, *dp, w, cur, step);
+    int w = b < n ? wl[xcd_remap(b, n)] : 0;
+    for (; b < n; b += G) {
+        const int wn = b + G < n ? wl[xcd_remap(b + G, n)] : 0;
+        const InstDesc& d = inst[MULTI ? tinst[w] : 0];
+        LowFetch<TS> f;
+        low_fetch<TS>(f, a, d, w);
+        sf_tile_low<TS>(L, a, d, w, f, step);
         __syncthreads();
-        if (w1 < 0) break;
-        cur = nxt;
-        w = w1;
-        dp = dp1;
-        w1 = w2;
-        b += G;
+        w = wn;
     }
 }
 
 // ------------------------------------------------------------ host launch
-int launch_sf_step(const StepArgs& a, int tile, bool adaptive, int grid_hi, int grid_lo, hipStream_t s) {
+// Persistent grid of kernel K: resident workgroups per CU (occupancy query,
+// cached per kernel) x CUs, at most one workgroup per work item.
+template <auto K>
+static int resident_grid(int nwork) {
+    static int per_cu = 0, cus = 0;
+    if (per_cu == 0) {
+        int v = 0, dev = 0, c = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, K, WG, 0) != hipSuccess || v <= 0) v = 2;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0)
+            c = 256;
+        cus = c;
+        per_cu = v;
+    }
+    const long long g = (long long)per_cu * cus;
+    return (int)(g < nwork ? g : nwork);
+}
+
+template <int TS, bool MULTI>
+static void sf_launch(const StepArgs& a, bool adaptive, hipStream_t s) {
     dim3 blk(WG);
+    if (adaptive) {
+        constexpr auto kh = sf_high_kernel<TS, true, MULTI>;
+        constexpr auto kl = sf_low_kernel<TS, MULTI>;
+        hipLaunchKernelGGL(kh, dim3(resident_grid<kh>(a.total_tiles)), blk, 0, s, a, a.inst, a.tile_inst, a.worklist,
+                           a.wl_count);
+        hipLaunchKernelGGL(kl, dim3(resident_grid<kl>(a.total_tiles)), blk, 0, s, a, a.inst, a.tile_inst, a.wl_low,
+                           a.wl_low_count);
+    } else {
+        constexpr auto kh = sf_high_kernel<TS, false, MULTI>;
+        hipLaunchKernelGGL(kh, dim3(resident_grid<kh>(a.nwork)), blk, 0, s, a, a.inst, a.tile_inst,
+                           (const int*)nullptr, (const int*)nullptr);
+    }
+}
+
+int launch_sf_step(const StepArgs& a, int tile, bool adaptive, hipStream_t s) {
+    const bool multi = a.ninst > 1;
     if (tile == 16) {
-        if (adaptive) {
-            hipLaunchKernelGGL((sf_high_kernel<16, true>), dim3(grid_hi), blk, 0, s, a);
-            hipLaunchKernelGGL((sf_low_kernel<16>), dim3(grid_lo), blk, 0, s, a);
-        } else {
-            hipLaunchKernelGGL((sf_high_kernel<16, false>), dim3(grid_hi), blk, 0, s, a);
-        }
+        if (multi) sf_launch<16, true>(a, adaptive, s);
+        else sf_launch<16, false>(a, adaptive, s);
     } else if (tile == 32) {
-        if (adaptive) {
-            hipLaunchKernelGGL((sf_high_kernel<32, true>), dim3(grid_hi), blk, 0, s, a);
-            hipLaunchKernelGGL((sf_low_kernel<32>), dim3(grid_lo), blk, 0, s, a);
-        } else {
-            hipLaunchKernelGGL((sf_high_kernel<32, false>), dim3(grid_hi), blk, 0, s, a);
-        }
+        if (multi) sf_launch<32, true>(a, adaptive, s);
+        else sf_launch<32, false>(a, adaptive, s);
     } else {
         return AMPS_E_UNSUPPORTED;
     }
