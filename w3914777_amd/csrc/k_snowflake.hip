@@ -272,7 +272,7 @@ __device__ inline void sf_halo_map(SfHalo<TS>& hm, const InstDesc& d, int ox, in
 
 template <int TS, bool ADAPT>
 __device__ void sf_tile_f64(SfLds<TS, double>& L, const StepArgs& a, const InstDesc& d, int tile_x, int tile_y,
-                            long long step) {
+                            unsigned hmask, long long step) {
     using G = SfGeo<TS>;
     constexpr int H = G::H, LW = G::LW, R = G::R;
     const int ox = tile_x * TS, oy = tile_y * TS;
@@ -280,7 +280,8 @@ __device__ void sf_tile_f64(SfLds<TS, double>& L, const StepArgs& a, const InstD
     const int gcell0 = (oy + ly) * d.nx + ox + x0;
     const double* fphi = a.in.f64 + d.cell_base;
     const double* fT = a.in.f64 + a.total_cells + d.cell_base;
-    // own cells (vector loads) and halo elements, all loads in flight together
+    // own cells (vector loads) and halo elements, all loads in flight together;
+    // a halo element is read from the storage its tile's type says is current
     double pc[R], tc[R];
     if (R % 2 == 0) {
 #pragma unroll
@@ -296,15 +297,18 @@ __device__ void sf_tile_f64(SfLds<TS, double>& L, const StepArgs& a, const InstD
     SfHalo<TS> hm;
     sf_halo_map<TS>(hm, d, ox, oy);
     double hp[G::NHT], ht[G::NHT];
-    __half hph[G::NHT], hth[G::NHT];
+    unsigned short hph[G::NHT], hth[G::NHT];
+    const unsigned short* hphi = reinterpret_cast<const unsigned short*>(a.in.f16 + d.cell_base);
+    const unsigned short* hT = reinterpret_cast<const unsigned short*>(a.in.f16 + a.total_cells + d.cell_base);
 #pragma unroll
     for (int k = 0; k < G::NHT; ++k) {
         if (hm.off[k] >= 0) {
-            hp[k] = fphi[hm.off[k]];
-            ht[k] = fT[hm.off[k]];
-            if (ADAPT) {
-                hph[k] = a.in.f16[d.cell_base + hm.off[k]];
-                hth[k] = a.in.f16[a.total_cells + d.cell_base + hm.off[k]];
+            if (!ADAPT || ((hmask >> hm.reg[k]) & 1u)) {
+                hp[k] = fphi[hm.off[k]];
+                ht[k] = fT[hm.off[k]];
+            } else {
+                hph[k] = hphi[hm.off[k]];
+                hth[k] = hT[hm.off[k]];
             }
         }
     }
@@ -324,12 +328,12 @@ __device__ void sf_tile_f64(SfLds<TS, double>& L, const StepArgs& a, const InstD
     for (int k = 0; k < G::NHT; ++k) {
         if (hm.off[k] < 0) continue;
         const int r = hm.reg[k];
-        if (!ADAPT || L.nbf[0].type[r] == 2) {
+        if (!ADAPT || ((hmask >> r) & 1u)) {
             L.phi[hm.box[k]] = hp[k];
             L.T[hm.box[k]] = ht[k];
         } else {
-            L.phi[hm.box[k]] = L.nbf[0].off[r] + (double)__half2float(hph[k]) * (double)L.nbf[0].scl[r];
-            L.T[hm.box[k]] = L.nbf[1].off[r] + (double)__half2float(hth[k]) * (double)L.nbf[1].scl[r];
+            L.phi[hm.box[k]] = L.nbf[0].off[r] + (double)__half2float(__ushort_as_half(hph[k])) * (double)L.nbf[0].scl[r];
+            L.T[hm.box[k]] = L.nbf[1].off[r] + (double)__half2float(__ushort_as_half(hth[k])) * (double)L.nbf[1].scl[r];
         }
     }
     __syncthreads();
@@ -393,23 +397,22 @@ __device__ inline void sf_store_low(__half* dst16, int gcell0, const unsigned sh
     }
 }
 
-// Everything a low tile needs from memory, fetched in one round so that the
-// fetch of the next tile of a workgroup overlaps the current tile's compute.
+// Everything a low tile needs from memory, fetched in one round: own fp16
+// words, the halo elements (each from the storage its tile's type in `hmask`
+// says is current) and the offsets/scales of the tile and its neighbours.
 template <int TS>
 struct LowFetch {
     using G = SfGeo<TS>;
     unsigned short op[G::R], ot[G::R];              // own fp16 words (phi, T)
-    double hp[G::NHT], ht[G::NHT];                  // halo FP64 candidates
-    unsigned short hph[G::NHT], hth[G::NHT];        // halo fp16 candidates
+    double hp[G::NHT], ht[G::NHT];                  // halo values: FP64, or the fp16 word's bits
     double offp, offT;
     float sclp, sclT;
-    int nbt;                                        // neighbour k = threadIdx.x (< 9)
     double nbo0, nbo1;
     float nbs0, nbs1;
 };
 
 template <int TS>
-__device__ inline void low_fetch(LowFetch<TS>& f, const StepArgs& a, const InstDesc& d, int w) {
+__device__ inline void low_fetch(LowFetch<TS>& f, const StepArgs& a, const InstDesc& d, int w, unsigned hmask) {
     using G = SfGeo<TS>;
     constexpr int R = G::R;
     const int lt = w - d.tile_base;
@@ -435,10 +438,13 @@ __device__ inline void low_fetch(LowFetch<TS>& f, const StepArgs& a, const InstD
 #pragma unroll
     for (int k = 0; k < G::NHT; ++k) {
         if (hm.off[k] >= 0) {
-            f.hp[k] = fphi[hm.off[k]];
-            f.ht[k] = fT[hm.off[k]];
-            f.hph[k] = hphi[hm.off[k]];
-            f.hth[k] = hT[hm.off[k]];
+            if ((hmask >> hm.reg[k]) & 1u) {
+                f.hp[k] = fphi[hm.off[k]];
+                f.ht[k] = fT[hm.off[k]];
+            } else {
+                f.hp[k] = __longlong_as_double((long long)hphi[hm.off[k]]);
+                f.ht[k] = __longlong_as_double((long long)hT[hm.off[k]]);
+            }
         }
     }
     f.offp = a.in.off[w];
@@ -449,7 +455,6 @@ __device__ inline void low_fetch(LowFetch<TS>& f, const StepArgs& a, const InstD
         Nb nb;
         nb_setup(nb, d, tile_x, tile_y, TS);
         const int g = nb_tile(nb, threadIdx.x);
-        f.nbt = a.type[g];
         f.nbo0 = a.in.off[g];
         f.nbo1 = a.in.off[a.total_tiles + g];
         f.nbs0 = a.in.scl[g];
@@ -459,7 +464,7 @@ __device__ inline void low_fetch(LowFetch<TS>& f, const StepArgs& a, const InstD
 
 template <int TS>
 __device__ void sf_tile_low(SfLds<TS, float>& L, const StepArgs& a, const InstDesc& d, int gtile,
-                            const LowFetch<TS>& f, long long step) {
+                            const LowFetch<TS>& f, unsigned hmask, long long step) {
     using G = SfGeo<TS>;
     constexpr int H = G::H, LW = G::LW, R = G::R;
     const int lt = gtile - d.tile_base;
@@ -470,8 +475,6 @@ __device__ void sf_tile_low(SfLds<TS, float>& L, const StepArgs& a, const InstDe
     const double offp = f.offp, offT = f.offT;
     const float sclp = f.sclp, sclT = f.sclT;
     if (threadIdx.x < 9) {
-        L.nbf[0].type[threadIdx.x] = f.nbt;
-        L.nbf[1].type[threadIdx.x] = f.nbt;
         L.nbf[0].off[threadIdx.x] = f.nbo0;
         L.nbf[1].off[threadIdx.x] = f.nbo1;
         L.nbf[0].scl[threadIdx.x] = f.nbs0;
@@ -500,12 +503,14 @@ __device__ void sf_tile_low(SfLds<TS, float>& L, const StepArgs& a, const InstDe
         if (hm.off[k] < 0) continue;
         const int r = hm.reg[k];
         float vp;
-        if (L.nbf[0].type[r] == 2) {
+        if ((hmask >> r) & 1u) {
             vp = (float)(f.hp[k] - offp);
             L.T[hm.box[k]] = (float)(f.ht[k] - offT);
         } else {
-            vp = (float)(L.nbf[0].off[r] - offp) + __half2float(__ushort_as_half(f.hph[k])) * L.nbf[0].scl[r];
-            L.T[hm.box[k]] = (float)(L.nbf[1].off[r] - offT) + __half2float(__ushort_as_half(f.hth[k])) * L.nbf[1].scl[r];
+            const unsigned short wp = (unsigned short)__double_as_longlong(f.hp[k]);
+            const unsigned short wt = (unsigned short)__double_as_longlong(f.ht[k]);
+            vp = (float)(L.nbf[0].off[r] - offp) + __half2float(__ushort_as_half(wp)) * L.nbf[0].scl[r];
+            L.T[hm.box[k]] = (float)(L.nbf[1].off[r] - offT) + __half2float(__ushort_as_half(wt)) * L.nbf[1].scl[r];
         }
         L.phi[hm.box[k]] = vp;
         near0 = near0 && fabsf(c0 + vp) <= SF_BULK_EPS;
@@ -587,6 +592,25 @@ __device__ void sf_tile_low(SfLds<TS, float>& L, const StepArgs& a, const InstDe
 // vector loads and waits per tile, and with MULTI = false (one instance) the
 // descriptor is loaded once per workgroup.  The next work-list entry is read
 // one tile ahead.
+// Type of neighbour tile (lane % 64) of tile w, for lanes 0..8 of every wave
+// (1 elsewhere); hmask_of() turns it into the wave-uniform 9-bit mask of FP64
+// neighbours.  Issued one tile ahead, so the halo loads of a tile never wait
+// on a type load.
+template <int TS>
+__device__ inline int nb_type_lane(const StepArgs& a, const InstDesc& d, int w) {
+    const int lane = threadIdx.x & 63;
+    if (lane >= 9 || a.type == nullptr) return 2;
+    const int lt = w - d.tile_base;
+    const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
+    Nb nb;
+    nb_setup(nb, d, tile_x, tile_y, TS);
+    return a.type[nb_tile(nb, lane)];
+}
+
+__device__ inline unsigned hmask_of(int t) {
+    return (unsigned)(__ballot((threadIdx.x & 63) < 9 && t == 2) & 0x1FFull);
+}
+
 template <int TS, bool ADAPT, bool MULTI>
 __global__ __launch_bounds__(256, 3) void sf_high_kernel(StepArgs a, const InstDesc* __restrict__ inst,
                                                          const int* __restrict__ tinst, const int* __restrict__ wl,
@@ -595,15 +619,20 @@ __global__ __launch_bounds__(256, 3) void sf_high_kernel(StepArgs a, const InstD
     const long long step = *a.step_ctr + a.step_add;
     const int n = wlc ? *wlc : a.nwork, G = gridDim.x;
     int b = blockIdx.x;
-    int w = b < n ? (wl ? wl[xcd_remap(b, n)] : xcd_remap(b, n)) : 0;
+    if (b >= n) return;
+    int w = wl ? wl[xcd_remap(b, n)] : xcd_remap(b, n);
+    int tcur = ADAPT ? nb_type_lane<TS>(a, inst[MULTI ? tinst[w] : 0], w) : 2;
     for (; b < n; b += G) {
-        const int wn = b + G < n ? (wl ? wl[xcd_remap(b + G, n)] : xcd_remap(b + G, n)) : 0;
+        const bool more = b + G < n;
+        const int wn = more ? (wl ? wl[xcd_remap(b + G, n)] : xcd_remap(b + G, n)) : w;
+        const int tnext = ADAPT && more ? nb_type_lane<TS>(a, inst[MULTI ? tinst[wn] : 0], wn) : 2;
         const InstDesc& d = inst[MULTI ? tinst[w] : 0];
         const int lt = w - d.tile_base;
         const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
-        sf_tile_f64<TS, ADAPT>(L, a, d, tile_x, tile_y, step);
+        sf_tile_f64<TS, ADAPT>(L, a, d, tile_x, tile_y, ADAPT ? hmask_of(tcur) : 0x1FFu, step);
         __syncthreads();
         w = wn;
+        tcur = tnext;
     }
 }
 
@@ -615,15 +644,21 @@ __global__ __launch_bounds__(256) void sf_low_kernel(StepArgs a, const InstDesc*
     const long long step = *a.step_ctr + a.step_add;
     const int n = *wlc, G = gridDim.x;
     int b = blockIdx.x;
-    int w = b < n ? wl[xcd_remap(b, n)] : 0;
+    if (b >= n) return;
+    int w = wl[xcd_remap(b, n)];
+    int tcur = nb_type_lane<TS>(a, inst[MULTI ? tinst[w] : 0], w);
     for (; b < n; b += G) {
-        const int wn = b + G < n ? wl[xcd_remap(b + G, n)] : 0;
+        const bool more = b + G < n;
+        const int wn = more ? wl[xcd_remap(b + G, n)] : w;
+        const int tnext = more ? nb_type_lane<TS>(a, inst[MULTI ? tinst[wn] : 0], wn) : 2;
         const InstDesc& d = inst[MULTI ? tinst[w] : 0];
+        const unsigned hmask = hmask_of(tcur);
         LowFetch<TS> f;
-        low_fetch<TS>(f, a, d, w);
-        sf_tile_low<TS>(L, a, d, w, f, step);
+        low_fetch<TS>(f, a, d, w, hmask);
+        sf_tile_low<TS>(L, a, d, w, f, hmask, step);
         __syncthreads();
         w = wn;
+        tcur = tnext;
     }
 }
 
