@@ -678,6 +678,7 @@ int amps_set_field(amps_ctx* c, int inst, int field, const double* src, size_t n
     if (inst < 0 || inst >= c->ninst || field < 0 || field >= c->nfields) return fail(AMPS_E_ARG, "instance or field out of range");
     const InstDesc& d = c->desc[inst];
     if (n != (size_t)d.nx * d.ny) return fail(AMPS_E_ARG, "n must equal nx*ny of the instance");
+    tape_clear(c);  // a recorded trajectory no longer leads to the new state
     double* dstp = c->buf[c->cur].f64 + (size_t)field * c->total_cells + d.cell_base;
     HIPCHK(hipMemcpyAsync(dstp, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     if (has_low(c) && c->gm.fg[field] >= 0) {
@@ -794,12 +795,151 @@ extern "C" int amps_get_kernel_time(amps_ctx* c, int64_t* launches, double* tota
 
 // ================================================================ adjoint
 extern "C" int amps_tape_begin(amps_ctx* c, int ckpt_every) {
-    (void)c; (void)ckpt_every;
-    return fail(AMPS_E_UNSUPPORTED, "adjoint not built yet");
+    if (!c || !c->initialised) return fail(AMPS_E_STATE, "context not initialised");
+    if (ckpt_every < 1) return fail(AMPS_E_ARG, "ckpt_every must be >= 1");
+    if (c->model != AMPS_SNOWFLAKE && c->model != AMPS_SINTERING)
+        return fail(AMPS_E_UNSUPPORTED, "the adjoint covers the snowflake and sintering models");
+    if (c->mode != AMPS_PURE_FP64) return fail(AMPS_E_UNSUPPORTED, "the adjoint needs a PURE_FP64 context");
+    tape_clear(c);
+    RCCHK(flush_adv(c));
+    c->tape_every = ckpt_every;
+    c->taping = true;
+    c->tape_step0 = c->steps_done;
+    return tape_push(c);
 }
+
+namespace {
+// Gradient slot of a parameter id for the model, or -1.
+int grad_slot(int model, int id) {
+    if (model == AMPS_SNOWFLAKE) {
+        switch (id) {
+            case AMPS_GRAD_SF_DELTA: return 0;
+            case AMPS_GRAD_SF_KAPPA: return 1;
+            case AMPS_GRAD_SF_EPSB: return 2;
+            case AMPS_GRAD_SF_TAU: return 3;
+        }
+    } else if (model == AMPS_SINTERING) {
+        switch (id) {
+            case AMPS_GRAD_SN_DSURF: return 0;
+            case AMPS_GRAD_SN_DGB: return 1;
+            case AMPS_GRAD_SN_KC: return 2;
+            case AMPS_GRAD_SN_KE: return 3;
+            case AMPS_GRAD_SN_L: return 4;
+            case AMPS_GRAD_SN_DVOL: return 5;
+        }
+    }
+    return -1;
+}
+
+// Device allocations of one adjoint call, released on every exit path.
+struct AdjMem {
+    std::vector<void*> p;
+    ~AdjMem() {
+        for (void* q : p) (void)hipFree(q);
+    }
+    int get(void** out, size_t bytes) {
+        *out = nullptr;
+        if (hipMalloc(out, bytes) != hipSuccess) return fail(AMPS_E_NOMEM, "adjoint work memory");
+        p.push_back(*out);
+        return AMPS_OK;
+    }
+};
+}  // namespace
 
 extern "C" int amps_adjoint(amps_ctx* c, const double* dJ_du, const int32_t* param_ids, int np, double* dJ_dp,
                             double* dJ_du0) {
-    (void)c; (void)dJ_du; (void)param_ids; (void)np; (void)dJ_dp; (void)dJ_du0;
-    return fail(AMPS_E_UNSUPPORTED, "adjoint not built yet");
+    if (!c || !c->initialised) return fail(AMPS_E_STATE, "context not initialised");
+    if (!c->taping || c->tape.empty()) return fail(AMPS_E_STATE, "amps_tape_begin has not been called");
+    if (!dJ_du) return fail(AMPS_E_ARG, "dJ_du is NULL");
+    if (np < 0 || (np > 0 && (!param_ids || !dJ_dp))) return fail(AMPS_E_ARG, "param_ids/dJ_dp missing");
+    std::vector<int> slot(np);
+    for (int k = 0; k < np; ++k) {
+        slot[k] = grad_slot(c->model, param_ids[k]);
+        if (slot[k] < 0) return fail(AMPS_E_ARG, "parameter id not differentiable for this model");
+    }
+    RCCHK(flush_adv(c));
+    const int NG = 6;
+    const int F = c->nfields;
+    const size_t TC = (size_t)c->total_cells, state = (size_t)F * TC;
+    const int k_every = c->tape_every;
+    AdjMem mem;
+    double* lam[2];
+    double* w[6];
+    double* gacc;
+    long long* dstep;
+    std::vector<double*> seg(k_every + 1);
+    RCCHK(mem.get((void**)&lam[0], state * 8));
+    RCCHK(mem.get((void**)&lam[1], state * 8));
+    for (int i = 0; i < 6; ++i) RCCHK(mem.get((void**)&w[i], TC * 8));
+    RCCHK(mem.get((void**)&gacc, (size_t)NG * c->total_tiles * 8));
+    RCCHK(mem.get((void**)&dstep, 8));
+    for (int j = 1; j <= k_every; ++j) RCCHK(mem.get((void**)&seg[j], state * 8));
+    // cotangent of the final state: host [inst][field][ny][nx] -> device [field][cells]
+    size_t hoff = 0;
+    for (int i = 0; i < c->ninst; ++i) {
+        const InstDesc& d = c->desc[i];
+        const size_t n = (size_t)d.nx * d.ny;
+        for (int f = 0; f < F; ++f, hoff += n)
+            HIPCHK(hipMemcpyAsync(lam[0] + f * TC + d.cell_base, dJ_du + hoff, n * 8, hipMemcpyHostToDevice,
+                                  c->stream));
+    }
+    HIPCHK(hipMemsetAsync(gacc, 0, (size_t)NG * c->total_tiles * 8, c->stream));
+    int cur = 0;
+    for (int s = (int)c->tape.size() - 1; s >= 0; --s) {
+        const long long S0 = c->tape_steps[s];
+        const long long S1 = s + 1 < (int)c->tape.size() ? c->tape_steps[s + 1] : c->steps_done;
+        const int m = (int)(S1 - S0);
+        if (m <= 0) continue;
+        // recompute the segment's states from its checkpoint
+        seg[0] = c->tape[s];
+        RCCHK(launch_fill_u64((unsigned long long*)dstep, 1, (unsigned long long)S0, c->stream));
+        for (int j = 0; j < m; ++j) {
+            StepArgs a = step_args(c, 0);
+            a.in.f64 = seg[j];
+            a.out.f64 = seg[j + 1];
+            a.step_ctr = dstep;
+            a.step_add = j;
+            RCCHK(launch_step_kernel(c, a));
+        }
+        for (int j = m - 1; j >= 0; --j) {
+            AdjArgs a{};
+            a.inst = c->d_desc;
+            a.tile_inst = c->d_tile_inst;
+            a.total_tiles = c->total_tiles;
+            a.total_cells = c->total_cells;
+            a.u = seg[j];
+            a.un = seg[j + 1];
+            a.lam_n = lam[cur];
+            a.lam = lam[cur ^ 1];
+            a.w0 = w[0]; a.w1 = w[1]; a.w2 = w[2]; a.w3 = w[3]; a.w4 = w[4]; a.w5 = w[5];
+            a.gacc = gacc;
+            a.step_ctr = dstep;
+            a.step_add = j;
+            RCCHK(launch_adjoint_step(a, c->model, c->tile, c->stream));
+            cur ^= 1;
+        }
+    }
+    std::vector<double> g((size_t)NG * c->total_tiles);
+    HIPCHK(hipMemcpyAsync(g.data(), gacc, g.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    if (dJ_du0) {
+        hoff = 0;
+        for (int i = 0; i < c->ninst; ++i) {
+            const InstDesc& d = c->desc[i];
+            const size_t n = (size_t)d.nx * d.ny;
+            for (int f = 0; f < F; ++f, hoff += n)
+                HIPCHK(hipMemcpyAsync(dJ_du0 + hoff, lam[cur] + f * TC + d.cell_base, n * 8, hipMemcpyDeviceToHost,
+                                      c->stream));
+        }
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < c->ninst; ++i) {
+        const InstDesc& d = c->desc[i];
+        const int nt = d.tx * d.ty;
+        for (int k = 0; k < np; ++k) {
+            double sum = 0.0;
+            for (int t = 0; t < nt; ++t) sum += g[(size_t)slot[k] * c->total_tiles + d.tile_base + t];
+            dJ_dp[(size_t)i * np + k] = sum;
+        }
+    }
+    return AMPS_OK;
 }

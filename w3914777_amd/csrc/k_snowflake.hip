@@ -30,53 +30,9 @@
 // rows are padded so that the update's neighbour reads are 16-byte aligned.
 #include "kernels.hpp"
 #include "tile_common.hpp"
+#include "sf_math.hpp"
 
 namespace amps {
-
-__device__ inline double rcp_any(double x) { return rcp_nr(x); }
-__device__ inline float rcp_any(float x) { return __builtin_amdgcn_rcpf(x); }
-
-template <typename R>
-__device__ inline void sf_aniso(R gx, R gy, int n, R cth0, R sth0, R epsb, R epsb_delta, R m_epsb_n_delta,
-                                R& eps, R& epsd) {
-    R mx = fmax(fabs(gx), fabs(gy));
-    R c = R(1), s = R(0);
-    if (mx != R(0)) {
-        int e;
-        frexp(mx, &e);
-        R u = ldexp(gx, -e), v = ldexp(gy, -e);  // exact scaling, max(|u|,|v|) in [0.5, 1)
-        R wr = R(1), wi = R(0), br = u, bi = v;
-        int k = n;
-        while (true) {
-            if (k & 1) {
-                R t = wr * br - wi * bi;
-                wi = wr * bi + wi * br;
-                wr = t;
-            }
-            k >>= 1;
-            if (!k) break;
-            R t = br * br - bi * bi;
-            bi = R(2) * br * bi;
-            br = t;
-        }
-        R r2 = u * u + v * v;
-        R pw = R(1), b = r2;
-        int h = n >> 1;
-        while (h) {
-            if (h & 1) pw *= b;
-            h >>= 1;
-            if (h) b *= b;
-        }
-        if (n & 1) pw *= sqrt(r2);
-        R inv = rcp_any(pw);
-        c = wr * inv;
-        s = wi * inv;
-    }
-    R ca = c * cth0 + s * sth0;  // cos(n (theta - theta0))
-    R sa = s * cth0 - c * sth0;  // sin(n (theta - theta0))
-    eps = epsb + epsb_delta * ca;
-    epsd = m_epsb_n_delta * sa;
-}
 
 // Geometry of a snowflake tile of TS x TS cells.
 template <int TS>
@@ -637,7 +593,7 @@ __global__ __launch_bounds__(256, 3) void sf_high_kernel(StepArgs a, const InstD
 }
 
 template <int TS, bool MULTI>
-__global__ __launch_bounds__(256) void sf_low_kernel(StepArgs a, const InstDesc* __restrict__ inst,
+__global__ __launch_bounds__(256, 5) void sf_low_kernel(StepArgs a, const InstDesc* __restrict__ inst,
                                                      const int* __restrict__ tinst, const int* __restrict__ wl,
                                                      const int* __restrict__ wlc) {
     __shared__ __align__(16) SfLds<TS, float> L;

@@ -36,4 +36,27 @@ int launch_gg_step(const StepArgs& a, int tile, bool adaptive, int grid, hipStre
 int launch_gg_aux(const StepArgs& a, int tile, hipStream_t s);
 int launch_sn_step(const StepArgs& a, int tile, bool adaptive, int grid, hipStream_t s);
 
+// Reverse-time adjoint of one FP64 step (snowflake, sintering).  Arrays are
+// [field][total_cells] like the FP64 state; work arrays hold total_cells.
+struct AdjArgs {
+    const InstDesc* inst;
+    const int* tile_inst;
+    int total_tiles;
+    long long total_cells;
+    const double* u;       // state before the step
+    const double* un;      // state after the step
+    const double* lam_n;   // dJ/d(state after)
+    double* lam;           // dJ/d(state before), written
+    double* w0;            // work arrays, one value per cell
+    double* w1;
+    double* w2;
+    double* w3;
+    double* w4;
+    double* w5;
+    double* gacc;          // [gradient slot][total_tiles] accumulators
+    const long long* step_ctr;  // step index of the forward step = *step_ctr + step_add
+    int step_add;
+};
+int launch_adjoint_step(const AdjArgs& a, int model, int tile, hipStream_t s);
+
 }  // namespace amps
