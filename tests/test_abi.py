@@ -121,3 +121,27 @@ def test_batch_must_share_model_and_mode(amps):
     g.q = 1
     with pytest.raises(amps.AmpsError):
         amps.Context(g)
+
+
+def test_autograd_binding_maps_every_gradient_id():
+    """The torch binding knows the parameter slot of every differentiable id
+    declared in amps.h and rejects other ids (no GPU needed)."""
+    import w3914777_amd as amps
+    from w3914777_amd.autograd import PARAM_SLOT, config_with
+    assert set(PARAM_SLOT) == set(amps.GRAD_SF.values()) | set(amps.GRAD_SN.values())
+    c = config_with(amps.Config.default(amps.SNOWFLAKE), [amps.GRAD_SF["DELTA"]], [0.07])
+    assert c.params[amps.SF["DELTA"]] == 0.07 and c.mode == amps.PURE_FP64
+    with pytest.raises(ValueError):
+        config_with(amps.Config.default(amps.SNOWFLAKE), [999], [1.0])
+
+
+def test_driver_usage_errors():
+    """The command-line driver is built and rejects bad invocations before
+    touching a GPU (exit status 2, usage on stderr)."""
+    drv = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "w3914777_amd", "bin", "amps_run")
+    assert os.access(drv, os.X_OK)
+    for args in ([], ["snowflake"], ["nosuchmodel", "purehigh", "1", "x.csv"], ["snowflake", "purehigh", "1"],
+                 ["sintering", "purehigh", "1", "x.csv"], ["graingrowth", "puremix"], ["snowflake", "main"],
+                 ["snowflake", "purelow", "--bogus"]):
+        r = subprocess.run([drv, *args], capture_output=True, text=True, timeout=30)
+        assert r.returncode == 2 and "usage" in r.stderr, (args, r.returncode, r.stderr)

@@ -2,7 +2,9 @@
 
 Every .hip/.cpp file under csrc/ is compiled to an object under _build/ and
 linked into w3914777_amd/libamps.so.  Packed FP32 instructions are disabled
-for all device code (-packed-fp32-ops).
+for all device code (-packed-fp32-ops).  The command-line drivers
+(drivers/amps_run.cpp, host code over the C ABI) are linked against it into
+w3914777_amd/bin/amps_run.
 """
 import concurrent.futures as cf
 import os
@@ -13,6 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libamps.so")
+DRIVER_SRC = os.path.join(HERE, "drivers", "amps_run.cpp")
+DRIVER = os.path.join(HERE, "bin", "amps_run")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
@@ -52,7 +56,21 @@ def build(verbose=False):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link failed")
     os.replace(tmp, LIB)
+    build_driver()
     return LIB
+
+
+def build_driver():
+    os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
+    tmp = DRIVER + ".tmp"
+    cmd = [HIPCC, "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(os.path.dirname(HERE), "include"), DRIVER_SRC,
+           "-L" + HERE, "-lamps", "-Wl,-rpath,$ORIGIN/..", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("driver build failed")
+    os.replace(tmp, DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":
