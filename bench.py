@@ -35,8 +35,12 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_path(amps, cfg, warmup, steps, kernel_steps):
-    """Returns dict with wall/event timings, final fields, info and sweep-kernel timing."""
+def run_path(amps, cfg, warmup, steps, kernel_pass=True):
+    """Timed run (graph replay, wall clock and HIP events around the step
+    loop), then -- for the roofline -- the same warmup + steps again from the
+    same initial state with HIP events around every sweep kernel (graph
+    replay suspended), so the per-kernel averages cover the same trajectory
+    as the timed region."""
     with amps.Context(cfg) as ctx:
         ctx.init()
         ctx.step(warmup)
@@ -47,16 +51,44 @@ def run_path(amps, cfg, warmup, steps, kernel_steps):
         wall = time.perf_counter() - t0
         info = ctx.info()
         fields = [ctx.get_field(0), ctx.get_field(1)]
-        # sweep-kernel timing pass: events bracket every step kernel on the
-        # context's stream (graph replay suspended), same workload state
-        ctx.set_kernel_timing(True)
-        ctx.step(kernel_steps)
-        ctx.sync()
-        nk, kms = ctx.kernel_time()
-        info2 = ctx.info()
-        ctx.set_kernel_timing(False)
-        return dict(wall=wall, ev_ms=ev_ms, info=info, fields=fields, k_launches=nk, k_ms=kms,
-                    hi_kernel=info2.hi_frac_last)
+    res = dict(wall=wall, ev_ms=ev_ms, info=info, fields=fields)
+    if kernel_pass:
+        with amps.Context(cfg) as ctx:
+            ctx.init()
+            ctx.step(warmup)
+            ctx.sync()
+            ctx.set_kernel_timing(True)
+            ctx.step(steps)
+            ctx.sync()
+            res["k_launches"], res["k_fp64_ms"] = ctx.sweep_time(amps.SWEEP_FP64)
+            _, res["k_low_ms"] = ctx.sweep_time(amps.SWEEP_LOW)
+            res["k_info"] = ctx.info()
+    return res
+
+
+def copy_probe():
+    """A plain device copy of known size (1 GiB read + 1 GiB write), the
+    calibration reference of the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = 1 << 27
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the low-precision sweep from the committed
+    rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy
+    kernel), or None."""
+    path = os.path.join(ROOT, "profiles", "sf_low_kernel_4096_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
 
 
 def cpu_baseline(cfg, seconds_budget):
@@ -81,7 +113,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--kernel-steps", type=int, default=200)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run a 1 GiB device copy first (counter calibration under rocprofv3 --pmc)")
     ap.add_argument("--threshold", type=float, default=None)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -103,13 +136,15 @@ def main():
     f64_cfg = cfg.copy()
     f64_cfg.mode = amps.PURE_FP64
 
+    if args.copy_probe:
+        copy_probe()
     log(f"[bench] adaptive path: {cfg.nx}x{cfg.ny}, warmup {args.warmup}, steps {args.steps}")
-    amp = run_path(amps, amp_cfg, args.warmup, args.steps, args.kernel_steps)
+    amp = run_path(amps, amp_cfg, args.warmup, args.steps)
     log(f"[bench] adaptive: {amp['wall']:.3f} s wall, {amp['ev_ms']:.1f} ms events, hi_frac {amp['info'].hi_frac_mean:.4f}")
     f64 = None
     if not args.no_fp64:
         log("[bench] FP64 path")
-        f64 = run_path(amps, f64_cfg, args.warmup, args.steps, args.kernel_steps)
+        f64 = run_path(amps, f64_cfg, args.warmup, args.steps)
         log(f"[bench] fp64: {f64['wall']:.3f} s wall, {f64['ev_ms']:.1f} ms events")
 
     value = cells * args.steps / amp["wall"]
@@ -121,19 +156,33 @@ def main():
                                f"{cfg.threshold}, 5-point, zero flux, noise 0.01, {args.warmup}+{args.steps} steps",
                    "grid": [cfg.nx, cfg.ny], "tile": cfg.tile, "threshold": cfg.threshold},
     }
-    # roofline of the adaptive sweep kernel: algorithmic bytes per launch
-    hi = amp["hi_kernel"]
-    b_alg = hi * B_FP64 + (1 - hi) * B_FP16
-    k_avg_s = amp["k_ms"] / max(amp["k_launches"], 1) / 1e3
-    achieved = b_alg * cells / k_avg_s / 1e9
+    # roofline of the dominant sweep kernel of the adaptive step (the
+    # low-precision sweep): algorithmic bytes = 8 B per low cell-update, the
+    # low fraction of each launch averaged over the kernel-timed run
+    ki = amp["k_info"]
+    hi_m = ki.hi_frac_mean
+    nl = max(amp["k_launches"], 1)
+    low_s = amp["k_low_ms"] / nl / 1e3
+    hi_s = amp["k_fp64_ms"] / nl / 1e3
+    low_bytes = (1.0 - hi_m) * cells * B_FP16
+    hi_bytes = hi_m * cells * B_FP64
+    achieved = low_bytes / low_s / 1e9
+    tr = traffic_from_profiles()
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                       "kernel": "sf_step_kernel<32,true>", "avg_launch_us": k_avg_s * 1e6,
-                       "bytes_per_cell_update": b_alg, "hi_frac": hi}
+                       "frac": achieved / HBM_PEAK_GBS,
+                       "traffic": tr["bytes_per_launch"] if tr else None,
+                       "kernel": "sf_low_kernel<32,false>", "avg_launch_us": low_s * 1e6,
+                       "alg_bytes_per_launch": low_bytes, "low_frac": 1.0 - hi_m}
+    out["fp64_sweep_of_adaptive"] = {"kernel": "sf_high_kernel<32,true,false>", "avg_launch_us": hi_s * 1e6,
+                                     "alg_bytes_per_launch": hi_bytes,
+                                     "achieved_GBs": hi_bytes / max(hi_s, 1e-12) / 1e9}
+    out["adaptive_step_sweeps"] = {"avg_us": (low_s + hi_s) * 1e6,
+                                   "alg_bytes_per_step": low_bytes + hi_bytes,
+                                   "achieved_GBs": (low_bytes + hi_bytes) / (low_s + hi_s) / 1e9}
     out["hi_frac_mean"] = amp["info"].hi_frac_mean
     if f64 is not None:
         f64_value = cells * args.steps / f64["wall"]
-        kf_s = f64["k_ms"] / max(f64["k_launches"], 1) / 1e3
+        kf_s = f64["k_fp64_ms"] / max(f64["k_launches"], 1) / 1e3
         fa = B_FP64 * cells / kf_s / 1e9
         err = {}
         for name, a, b in (("phi", amp["fields"][0], f64["fields"][0]), ("T", amp["fields"][1], f64["fields"][1])):
@@ -144,7 +193,7 @@ def main():
         out["fp64"] = {"value": f64_value, "ms_per_step": f64["wall"] * 1e3 / args.steps,
                        "speedup_amp_over_fp64": value / f64_value,
                        "roofline": {"achieved": fa, "peak": HBM_PEAK_GBS, "frac": fa / HBM_PEAK_GBS,
-                                    "kernel": "sf_step_kernel<32,false>", "avg_launch_us": kf_s * 1e6}}
+                                    "kernel": "sf_high_kernel<32,false,false>", "avg_launch_us": kf_s * 1e6}}
         out["error_vs_fp64"] = err
     if not args.no_cpu:
         log("[bench] cpu baseline")

@@ -24,6 +24,7 @@ LIB_PATH = os.path.join(_HERE, "libamps.so")
 SNOWFLAKE, GRAINGROWTH, SINTERING = 1, 2, 3
 PURE_FP64, PURE_HALF2, ADAPTIVE, STATIC_MAP = 0, 1, 2, 3
 MON_NONE, MON_SPATIAL, MON_TEMPORAL, MON_GRADIENT = 0, 1, 2, 3
+SWEEP_ALL, SWEEP_FP64, SWEEP_LOW = 0, 1, 2
 STENCIL_REF, STENCIL_LAP5 = 0, 1
 BC_PERIODIC, BC_NEUMANN = 0, 1
 INIT_REF, INIT_CONFIG = 0, 1
@@ -96,7 +97,7 @@ _lib = None
 EXPORTS = ["amps_default_config", "amps_load_paras", "amps_create", "amps_create_batch", "amps_destroy",
            "amps_init", "amps_step", "amps_sync", "amps_monitor_now", "amps_get_field", "amps_set_field",
            "amps_get_tile_types", "amps_set_tile_types", "amps_get_monitor_values", "amps_get_info",
-           "amps_set_kernel_timing", "amps_get_kernel_time",
+           "amps_set_kernel_timing", "amps_get_kernel_time", "amps_get_sweep_time",
            "amps_tape_begin", "amps_adjoint", "amps_status_string", "amps_last_error"]
 
 
@@ -130,6 +131,7 @@ def lib():
     L.amps_get_info.argtypes = [P, ctypes.POINTER(Info)]
     L.amps_set_kernel_timing.argtypes = [P, ctypes.c_int]
     L.amps_get_kernel_time.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]
+    L.amps_get_sweep_time.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]
     L.amps_tape_begin.argtypes = [P, ctypes.c_int]
     L.amps_adjoint.argtypes = [P, D, I32P, ctypes.c_int, D, D]
     L.amps_status_string.argtypes = [ctypes.c_int]
@@ -249,6 +251,13 @@ class Context:
         n = ctypes.c_int64(0)
         ms = ctypes.c_double(0)
         _check(lib().amps_get_kernel_time(self._h, ctypes.byref(n), ctypes.byref(ms)))
+        return n.value, ms.value
+
+    def sweep_time(self, which):
+        """(launches, total_ms) of one sweep: SWEEP_ALL, SWEEP_FP64 or SWEEP_LOW."""
+        n = ctypes.c_int64(0)
+        ms = ctypes.c_double(0)
+        _check(lib().amps_get_sweep_time(self._h, int(which), ctypes.byref(n), ctypes.byref(ms)))
         return n.value, ms.value
 
     def tape_begin(self, ckpt_every):

@@ -104,10 +104,11 @@ struct amps_ctx {
 
     // sweep-kernel timing (events around each step-kernel launch)
     bool ktiming = false;
-    std::vector<hipEvent_t> kev;  // pairs
+    std::vector<hipEvent_t> kev;  // triples: before, between the two snowflake sweeps, after
     int kev_used = 0;
     long long klaunches = 0;
-    double kms = 0.0;
+    double kms = 0.0;                      // all sweep kernels
+    double kms_part[2] = {0.0, 0.0};       // snowflake adaptive: FP64 sweep, low sweep
 
     std::vector<void*> allocs;
 };
@@ -253,33 +254,40 @@ int kev_flush(amps_ctx* c) {
     if (c->kev_used == 0) return AMPS_OK;
     HIPCHK(hipStreamSynchronize(c->stream));
     for (int i = 0; i < c->kev_used; ++i) {
-        float ms = 0.0f;
-        HIPCHK(hipEventElapsedTime(&ms, c->kev[2 * i], c->kev[2 * i + 1]));
-        c->kms += ms;
+        float a = 0.0f, b = 0.0f;
+        HIPCHK(hipEventElapsedTime(&a, c->kev[3 * i], c->kev[3 * i + 1]));
+        HIPCHK(hipEventElapsedTime(&b, c->kev[3 * i + 1], c->kev[3 * i + 2]));
+        c->kms += (double)a + (double)b;
+        c->kms_part[0] += a;
+        c->kms_part[1] += b;
     }
     c->klaunches += c->kev_used;
     c->kev_used = 0;
     return AMPS_OK;
 }
 
-int launch_step_kernel(amps_ctx* c, const StepArgs& a) {
+int launch_step_kernel(amps_ctx* c, const StepArgs& a, hipEvent_t mid = nullptr) {
     const bool adaptive = has_low(c);
     const int grid = a.nwork;
-    if (c->model == AMPS_SNOWFLAKE)
-        return launch_sf_step(a, c->tile, adaptive, c->stream);
-    if (c->model == AMPS_GRAINGROWTH) return launch_gg_step(a, c->tile, adaptive, grid, c->stream);
-    return launch_sn_step(a, c->tile, adaptive, grid, c->stream);
+    if (c->model == AMPS_SNOWFLAKE) return launch_sf_step(a, c->tile, adaptive, c->stream, mid);
+    int rc = c->model == AMPS_GRAINGROWTH ? launch_gg_step(a, c->tile, adaptive, grid, c->stream)
+                                          : launch_sn_step(a, c->tile, adaptive, grid, c->stream);
+    if (rc == AMPS_OK && mid) HIPCHK(hipEventRecord(mid, c->stream));
+    return rc;
 }
 
-// Enqueue one explicit-Euler step reading buffer `from`.
+// Enqueue one explicit-Euler step reading buffer `from`.  With kernel timing
+// on, events bracket the sweep kernels (and separate the snowflake's FP64
+// and low-precision sweeps).
 int enqueue_step(amps_ctx* c, int from, int step_add, bool capturing = false) {
     StepArgs a = step_args(c, from);
     a.step_add = step_add;
     if (!c->ktiming || capturing) return launch_step_kernel(c, a);
-    if (c->kev_used * 2 + 2 > (int)c->kev.size()) RCCHK(kev_flush(c));
-    HIPCHK(hipEventRecord(c->kev[2 * c->kev_used], c->stream));
-    RCCHK(launch_step_kernel(c, a));
-    HIPCHK(hipEventRecord(c->kev[2 * c->kev_used + 1], c->stream));
+    if (c->kev_used * 3 + 3 > (int)c->kev.size()) RCCHK(kev_flush(c));
+    hipEvent_t* e = &c->kev[3 * c->kev_used];
+    HIPCHK(hipEventRecord(e[0], c->stream));
+    RCCHK(launch_step_kernel(c, a, e[1]));
+    HIPCHK(hipEventRecord(e[2], c->stream));
     c->kev_used++;
     return AMPS_OK;
 }
@@ -776,20 +784,26 @@ extern "C" int amps_set_kernel_timing(amps_ctx* c, int enable) {
     if (!c) return fail(AMPS_E_ARG, "ctx is NULL");
     RCCHK(kev_flush(c));
     if (enable && c->kev.empty()) {
-        c->kev.resize(2 * 512);
+        c->kev.resize(3 * 512);
         for (auto& e : c->kev) HIPCHK(hipEventCreate(&e));
     }
     c->ktiming = enable != 0;
     c->klaunches = 0;
     c->kms = 0.0;
+    c->kms_part[0] = c->kms_part[1] = 0.0;
     return AMPS_OK;
 }
 
 extern "C" int amps_get_kernel_time(amps_ctx* c, int64_t* launches, double* total_ms) {
+    return amps_get_sweep_time(c, AMPS_SWEEP_ALL, launches, total_ms);
+}
+
+extern "C" int amps_get_sweep_time(amps_ctx* c, int which, int64_t* launches, double* total_ms) {
     if (!c || !launches || !total_ms) return fail(AMPS_E_ARG, "NULL argument");
+    if (which < AMPS_SWEEP_ALL || which > AMPS_SWEEP_LOW) return fail(AMPS_E_ARG, "which out of range");
     RCCHK(kev_flush(c));
     *launches = c->klaunches;
-    *total_ms = c->kms;
+    *total_ms = which == AMPS_SWEEP_ALL ? c->kms : c->kms_part[which - 1];
     return AMPS_OK;
 }
 

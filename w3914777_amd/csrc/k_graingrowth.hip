@@ -27,7 +27,10 @@ struct GgLds {
         double d[LW * LW];
         float f[LW * LW];
     };
-    NbField nbf;
+    // precision state of the 3x3 tiles around the workgroup's tile, per field
+    int ntype[AMPS_MAX_FIELDS][9];
+    double noff[AMPS_MAX_FIELDS][9];
+    float nscl[AMPS_MAX_FIELDS][9];
     float red[8];
 };
 
@@ -41,13 +44,36 @@ __device__ inline T gg_lap(const T* B, int c, int LW, const T* cv, int i, bool m
     return (B[c - 1] + B[c + 1] + cv[i] + cv[i + 2] - T(4) * cv[i + 1]) * inv;
 }
 
+// Raw box values of field f: FP64 bits, or the fp16 word where the element's
+// tile is low (the type table in LDS says which storage is current).
+template <int TS, bool ADAPT>
+__device__ inline void gg_load_raw(unsigned long long* raw, const StepArgs& a, const InstDesc& d, int f,
+                                   const BoxMap<TS, 1>& m, const GgLds<TS>& L) {
+    constexpr int N = BoxMap<TS, 1>::N;
+    const double* f64 = a.in.f64 + (size_t)f * a.total_cells + d.cell_base;
+    const unsigned short* f16 =
+        reinterpret_cast<const unsigned short*>(a.in.f16 + (size_t)f * a.total_cells + d.cell_base);
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        if (m.off[k] >= 0) {
+            if (!ADAPT || L.ntype[f][m.reg[k]] == 2) raw[k] = (unsigned long long)__double_as_longlong(f64[m.off[k]]);
+            else raw[k] = f16[m.off[k]];
+        }
+    }
+}
+
 // Read-only metadata (descriptors, tile -> instance map, tile types) arrive
 // as restrict-qualified arguments so that the per-tile reads go through the
 // scalar cache; with MULTI = false the descriptor is loop-invariant.
+//
+// Per tile: one memory round brings the neighbour precision table of every
+// field, the carried sum of squares and field 0's box; afterwards the box of
+// field f+1 is loaded into registers while field f is computed, so each field
+// costs one overlapped load round.
 template <int TS, bool ADAPT, bool MULTI>
-__global__ __launch_bounds__(256) void gg_step_kernel(StepArgs a, const InstDesc* __restrict__ inst,
+__global__ __launch_bounds__(256, 4) void gg_step_kernel(StepArgs a, const InstDesc* __restrict__ inst,
                                                       const int* __restrict__ tinst, const int* __restrict__ types) {
-    constexpr int H = 1, LW = TS + 2, R = TS * TS / WG;
+    constexpr int H = 1, LW = TS + 2, R = TS * TS / WG, N = BoxMap<TS, 1>::N;
     __shared__ GgLds<TS> L;
     for (int b = blockIdx.x; b < a.nwork; b += gridDim.x) {
         const int w = xcd_remap(b, a.nwork);
@@ -55,8 +81,6 @@ __global__ __launch_bounds__(256) void gg_step_kernel(StepArgs a, const InstDesc
         const int lt = w - d.tile_base;
         const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
         const int ox = tile_x * TS, oy = tile_y * TS;
-        Nb nb;
-        nb_setup(nb, d, tile_x, tile_y, TS);
         BoxMap<TS, H> m;
         box_map<TS, H>(m, d, tile_x, tile_y);
         const double inv = d.k[GGK_INV_DXDY], dtl = d.k[GGK_DT_L], kap = d.k[GGK_KAPPA];
@@ -64,6 +88,18 @@ __global__ __launch_bounds__(256) void gg_step_kernel(StepArgs a, const InstDesc
         const bool moore = d.stencil == AMPS_STENCIL_REF;
         const int lx = threadIdx.x % TS, ly0 = (threadIdx.x / TS) * R;
 
+        if (ADAPT) {
+            Nb nb;
+            nb_setup(nb, d, tile_x, tile_y, TS);
+            for (int e = threadIdx.x; e < a.q * 9; e += WG) {
+                const int f = e / 9, k = e - f * 9;
+                const int g = nb_tile(nb, k);
+                L.ntype[f][k] = types[(size_t)f * a.total_tiles + g];
+                L.noff[f][k] = a.in.off[(size_t)f * a.total_tiles + g];
+                L.nscl[f][k] = a.in.scl[(size_t)f * a.total_tiles + g];
+            }
+            __syncthreads();
+        }
         double S[R], Sn[R];
         int gcell[R];
 #pragma unroll
@@ -72,15 +108,40 @@ __global__ __launch_bounds__(256) void gg_step_kernel(StepArgs a, const InstDesc
             S[i] = a.aux_in[d.cell_base + gcell[i]];
             Sn[i] = 0.0;
         }
+        unsigned long long raw[N];
+        gg_load_raw<TS, ADAPT>(raw, a, d, 0, m, L);
         for (int f = 0; f < a.q; ++f) {
-            const int ty = ADAPT ? types[(size_t)f * a.total_tiles + w] : 2;
-            const double* f64 = a.in.f64 + (size_t)f * a.total_cells + d.cell_base;
-            if (ADAPT) nbfield_fill(&L.nbf, a, a.in, nb, f, f);
+            const int ty = ADAPT ? L.ntype[f][4] : 2;
+            // stage field f from the raw values, then start loading field f+1
+            if (ty == 2) {
+#pragma unroll
+                for (int k = 0; k < N; ++k) {
+                    if (m.off[k] < 0) continue;
+                    const int r = m.reg[k];
+                    L.d[threadIdx.x + k * WG] =
+                        (!ADAPT || L.ntype[f][r] == 2)
+                            ? __longlong_as_double((long long)raw[k])
+                            : L.noff[f][r] + (double)__half2float(__ushort_as_half((unsigned short)raw[k])) *
+                                                 (double)L.nscl[f][r];
+                }
+            } else {
+                const double own_off = L.noff[f][4];
+#pragma unroll
+                for (int k = 0; k < N; ++k) {
+                    if (m.off[k] < 0) continue;
+                    const int r = m.reg[k];
+                    float x;
+                    if (L.ntype[f][r] == 2) x = (float)(__longlong_as_double((long long)raw[k]) - own_off);
+                    else if (r == 4) x = __half2float(__ushort_as_half((unsigned short)raw[k])) * L.nscl[f][4];
+                    else
+                        x = (float)(L.noff[f][r] - own_off) +
+                            __half2float(__ushort_as_half((unsigned short)raw[k])) * L.nscl[f][r];
+                    L.f[threadIdx.x + k * WG] = x;
+                }
+            }
+            if (f + 1 < a.q) gg_load_raw<TS, ADAPT>(raw, a, d, f + 1, m, L);
             __syncthreads();
             if (ty == 2) {
-                if (ADAPT) box_load_mixed_f64<TS, H>(L.d, f64, a.in.f16 + (size_t)f * a.total_cells + d.cell_base, &L.nbf, m);
-                else box_load1_f64<TS, H>(L.d, f64, m);
-                __syncthreads();
                 double* out = a.out.f64 + (size_t)f * a.total_cells + d.cell_base;
                 double cv[R + 2];
 #pragma unroll
@@ -97,11 +158,9 @@ __global__ __launch_bounds__(256) void gg_step_kernel(StepArgs a, const InstDesc
                     Sn[i] += en * en;
                 }
             } else {
-                const double own_off = L.nbf.off[4];
-                box_load_dev32<TS, H>(L.f, f64, a.in.f16 + (size_t)f * a.total_cells + d.cell_base, &L.nbf, own_off, m);
-                __syncthreads();
                 const float invf = d.kf[GGK_INV_DXDY], dtlf = d.kf[GGK_DT_L], kapf = d.kf[GGK_KAPPA];
                 const float alf = d.kf[GGK_ALPHA], bef = d.kf[GGK_BETA], g2f = d.kf[GGK_2GAMMA];
+                const double own_off = L.noff[f][4];
                 const float offf = (float)own_off;
                 float cv[R + 2];
 #pragma unroll
@@ -137,10 +196,10 @@ __global__ __launch_bounds__(256) void gg_step_kernel(StepArgs a, const InstDesc
                     a.out.scl[(size_t)f * a.total_tiles + w] = tc.scl;
                 }
             }
+            __syncthreads();
         }
 #pragma unroll
         for (int i = 0; i < R; ++i) a.aux_out[d.cell_base + gcell[i]] = Sn[i];
-        __syncthreads();
     }
 }
 

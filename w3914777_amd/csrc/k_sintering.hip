@@ -70,12 +70,40 @@ __global__ __launch_bounds__(256) void sn_step_kernel(StepArgs a, const InstDesc
         const double* f1 = a.in.f64 + a.total_cells + d.cell_base;
         const double* f2 = a.in.f64 + 2 * a.total_cells + d.cell_base;
         if (ADAPT) {
+            // one memory round: c (always FP64), the etas of the own tile from
+            // its current storage, the 1-cell halo ring from both storages
+            // (selected by the neighbour table loaded alongside)
+            constexpr int NC = BoxMap<TS, 2>::N, NE = BoxMap<TS, 1>::N;
             nbfield_fill(&L.nbf[0], a, a.in, nb, 1, 0);
             nbfield_fill(&L.nbf[1], a, a.in, nb, 2, 0);
+            const __half* h1 = a.in.f16 + a.total_cells + d.cell_base;
+            const __half* h2 = a.in.f16 + 2 * a.total_cells + d.cell_base;
+            double vc[NC], v1[NE], v2[NE];
+            __half q1[NE], q2[NE];
+#pragma unroll
+            for (int k = 0; k < NC; ++k) if (mc.off[k] >= 0) vc[k] = fc[mc.off[k]];
+#pragma unroll
+            for (int k = 0; k < NE; ++k) {
+                if (me.off[k] < 0) continue;
+                const bool own = me.reg[k] == 4;
+                if (!own || ty == 2) { v1[k] = f1[me.off[k]]; v2[k] = f2[me.off[k]]; }
+                if (!own || ty != 2) { q1[k] = h1[me.off[k]]; q2[k] = h2[me.off[k]]; }
+            }
             __syncthreads();
-            box_load1_f64<TS, 2>(L.c, fc, mc);
-            box_load_mixed_f64<TS, 1>(L.e1, f1, a.in.f16 + a.total_cells + d.cell_base, &L.nbf[0], me);
-            box_load_mixed_f64<TS, 1>(L.e2, f2, a.in.f16 + 2 * a.total_cells + d.cell_base, &L.nbf[1], me);
+#pragma unroll
+            for (int k = 0; k < NC; ++k) if (mc.off[k] >= 0) L.c[threadIdx.x + k * WG] = vc[k];
+#pragma unroll
+            for (int k = 0; k < NE; ++k) {
+                if (me.off[k] < 0) continue;
+                const int r = me.reg[k];
+                if (L.nbf[0].type[r] == 2) {
+                    L.e1[threadIdx.x + k * WG] = v1[k];
+                    L.e2[threadIdx.x + k * WG] = v2[k];
+                } else {
+                    L.e1[threadIdx.x + k * WG] = L.nbf[0].off[r] + (double)__half2float(q1[k]) * (double)L.nbf[0].scl[r];
+                    L.e2[threadIdx.x + k * WG] = L.nbf[1].off[r] + (double)__half2float(q2[k]) * (double)L.nbf[1].scl[r];
+                }
+            }
         } else {
             // all three fields' loads in flight together
             constexpr int NC = BoxMap<TS, 2>::N, NE = BoxMap<TS, 1>::N;

@@ -638,30 +638,32 @@ static int resident_grid(int nwork) {
 }
 
 template <int TS, bool MULTI>
-static void sf_launch(const StepArgs& a, bool adaptive, hipStream_t s) {
+static void sf_launch(const StepArgs& a, bool adaptive, hipStream_t s, hipEvent_t mid) {
     dim3 blk(WG);
     if (adaptive) {
         constexpr auto kh = sf_high_kernel<TS, true, MULTI>;
         constexpr auto kl = sf_low_kernel<TS, MULTI>;
         hipLaunchKernelGGL(kh, dim3(resident_grid<kh>(a.total_tiles)), blk, 0, s, a, a.inst, a.tile_inst, a.worklist,
                            a.wl_count);
+        if (mid) (void)hipEventRecord(mid, s);
         hipLaunchKernelGGL(kl, dim3(resident_grid<kl>(a.total_tiles)), blk, 0, s, a, a.inst, a.tile_inst, a.wl_low,
                            a.wl_low_count);
     } else {
         constexpr auto kh = sf_high_kernel<TS, false, MULTI>;
         hipLaunchKernelGGL(kh, dim3(resident_grid<kh>(a.nwork)), blk, 0, s, a, a.inst, a.tile_inst,
                            (const int*)nullptr, (const int*)nullptr);
+        if (mid) (void)hipEventRecord(mid, s);
     }
 }
 
-int launch_sf_step(const StepArgs& a, int tile, bool adaptive, hipStream_t s) {
+int launch_sf_step(const StepArgs& a, int tile, bool adaptive, hipStream_t s, hipEvent_t mid) {
     const bool multi = a.ninst > 1;
     if (tile == 16) {
-        if (multi) sf_launch<16, true>(a, adaptive, s);
-        else sf_launch<16, false>(a, adaptive, s);
+        if (multi) sf_launch<16, true>(a, adaptive, s, mid);
+        else sf_launch<16, false>(a, adaptive, s, mid);
     } else if (tile == 32) {
-        if (multi) sf_launch<32, true>(a, adaptive, s);
-        else sf_launch<32, false>(a, adaptive, s);
+        if (multi) sf_launch<32, true>(a, adaptive, s, mid);
+        else sf_launch<32, false>(a, adaptive, s, mid);
     } else {
         return AMPS_E_UNSUPPORTED;
     }

@@ -30,7 +30,8 @@ int launch_nonfinite(const double* v, long long n, unsigned long long* cnt, hipS
 int launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t s);
 int launch_advance_step(long long* step_ctr, int n, hipStream_t s);
 
-int launch_sf_step(const StepArgs& a, int tile, bool adaptive, hipStream_t s);
+// mid (may be null) is recorded between the FP64 and the low-precision sweep.
+int launch_sf_step(const StepArgs& a, int tile, bool adaptive, hipStream_t s, hipEvent_t mid = nullptr);
 int launch_worklist(const int* type, int total_tiles, int* wl_hi, int* wl_lo, int* counts, hipStream_t s);
 int launch_gg_step(const StepArgs& a, int tile, bool adaptive, int grid, hipStream_t s);
 int launch_gg_aux(const StepArgs& a, int tile, hipStream_t s);

@@ -154,6 +154,7 @@ void apply(amps_config* cfg, const std::map<std::string, Val>& sym, int* timeste
     if (get("tile", &v)) cfg->tile = (int)v;
     if (get("threshold", &v)) cfg->threshold = v;
     if (get("monitor_every", &v)) cfg->monitor_every = (int)v;
+    if (get("monitor_field", &v)) cfg->monitor_field = (int)v;
     if (get("Q", &v) || get("q", &v)) cfg->q = (int)v;
     if (get("rng_seed", &v)) cfg->seed = (uint64_t)v;
     if (get("timesteps", &v)) { if (timesteps) *timesteps = (int)v; *set_ts = true; }

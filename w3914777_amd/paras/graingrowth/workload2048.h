@@ -6,6 +6,7 @@ int timesteps=10000;
 const int dimX=2048,dimY=2048;
 const int unitx=32,unity=32;
 const int Q=36;
+const int monitor_field=-1;
 const highprecision threshold=0.05;
 #define AMSTENCIL
 #define MonitorGradient
