@@ -107,6 +107,9 @@ typedef struct {
     int32_t stencil;        /* amps_stencil                                        */
     int32_t bc;             /* amps_bc                                             */
     int32_t init;           /* amps_init_kind                                      */
+    int32_t wg_per_cu;      /* sweep workgroups per CU (sets tiles per workgroup);
+                               0 = as many as fit (occupancy maximum)              */
+    int32_t reserved;       /* must be 0                                           */
     double threshold;       /* monitor threshold                                   */
     uint64_t seed;          /* counter-based RNG seed (noise, random init)         */
     double params[AMPS_NPARAMS];
@@ -252,6 +255,27 @@ int amps_tape_begin(amps_ctx* ctx, int ckpt_every);
  * dJ_du.  Each segment between checkpoints is recomputed forward on the GPU. */
 int amps_adjoint(amps_ctx* ctx, const double* dJ_du, const int32_t* param_ids, int np,
                  double* dJ_dp, double* dJ_du0);
+
+/* ---- autotuning ---- */
+
+/* A tiling of a run: tile edge and sweep workgroups per CU (0 = occupancy
+ * maximum), with the time per step measured for it. */
+typedef struct {
+    int32_t tile;
+    int32_t wg_per_cu;
+    double us_per_step;
+} amps_tuning;
+
+/* Time `steps` explicit-Euler steps (after a short warm-up) of every candidate
+ * tiling of the run cfgs[0..n-1] -- tile 16 or 32 (where the grids allow) x
+ * workgroups per CU in {occupancy maximum, 1, 2, 3, 4} -- each on a scratch
+ * context started from the run's initial state, and return the fastest in
+ * *best.  cfgs' own tile and wg_per_cu are ignored; apply the result by
+ * setting them in the configs given to amps_create.  Results are cached for
+ * the process per (model, mode, monitor, grids, q, steps).  No reference
+ * counterpart: the reference hard-codes 16x16 tiles and 8x16-thread blocks
+ * (snowflake/paras/total.h:21, end2end_mix.cu:76-77). */
+int amps_autotune(const amps_config* cfgs, int n, int steps, amps_tuning* best);
 
 /* ---- diagnostics ---- */
 const char* amps_status_string(int status);

@@ -181,10 +181,11 @@ def test_adaptive_snowflake_error_vs_fp64(amps, O):
         assert np.max(np.abs(g - r)) < 0.05
 
 
-def test_adaptive_graingrowth_per_field_types(amps, O):
+@pytest.mark.parametrize("tile", [16, 32])
+def test_adaptive_graingrowth_per_field_types(amps, O, tile):
     n, q = 128, 4
     base = amps.Config.default(amps.GRAINGROWTH)
-    base.nx = base.ny = n; base.q = q; base.init = amps.INIT_CONFIG; base.tile = 16
+    base.nx = base.ny = n; base.q = q; base.init = amps.INIT_CONFIG; base.tile = tile
     p = O.gg_params_config()
     for i in range(12):
         base.params[i] = p[i]
@@ -197,7 +198,7 @@ def test_adaptive_graingrowth_per_field_types(amps, O):
         ctx.sync()
         got = [ctx.get_field(f) for f in range(q)]
         types = ctx.get_tile_types()
-    assert types.shape == (q, n // 16, n // 16)
+    assert types.shape == (q, n // tile, n // tile)
     err = max(l2_rms(g, r) for g, r in zip(got, ref))
     assert err < 2e-3, err
 

@@ -116,12 +116,13 @@ def test_graph_replay_equals_eager_bitwise(amps, O, model, monkeypatch):
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("q,stencil,bc", [(2, 0, 0), (5, 0, 1), (3, 1, 0)])
-def test_gg_fp64_step_matches_oracle(amps, O, q, stencil, bc):
+@pytest.mark.parametrize("q,stencil,bc,tile", [(2, 0, 0, 16), (5, 0, 1, 16), (3, 1, 0, 16), (4, 0, 1, 32),
+                                                (3, 1, 0, 32)])
+def test_gg_fp64_step_matches_oracle(amps, O, q, stencil, bc, tile):
     n = 64
     c = amps.Config.default(amps.GRAINGROWTH)
     c.nx = c.ny = n
-    c.q, c.stencil, c.bc = q, stencil, bc
+    c.q, c.stencil, c.bc, c.tile = q, stencil, bc, tile
     p = O.gg_params_ref(1, n, n)
     for i in range(12):
         c.params[i] = p[i]

@@ -55,6 +55,7 @@ class Config(ctypes.Structure):
                 ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("q", ctypes.c_int32), ("tile", ctypes.c_int32),
                 ("monitor_every", ctypes.c_int32), ("monitor_field", ctypes.c_int32),
                 ("stencil", ctypes.c_int32), ("bc", ctypes.c_int32), ("init", ctypes.c_int32),
+                ("wg_per_cu", ctypes.c_int32), ("reserved", ctypes.c_int32),
                 ("threshold", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("params", ctypes.c_double * NPARAMS)]
 
@@ -86,6 +87,11 @@ class Config(ctypes.Structure):
         return {SNOWFLAKE: 2, GRAINGROWTH: self.q, SINTERING: 3}[self.model]
 
 
+class Tuning(ctypes.Structure):
+    """Mirror of amps_tuning (include/amps.h)."""
+    _fields_ = [("tile", ctypes.c_int32), ("wg_per_cu", ctypes.c_int32), ("us_per_step", ctypes.c_double)]
+
+
 class Info(ctypes.Structure):
     _fields_ = [("steps_done", ctypes.c_int64), ("monitor_passes", ctypes.c_int64), ("ntiles", ctypes.c_int32),
                 ("type_groups", ctypes.c_int32), ("nfields", ctypes.c_int32), ("ninstances", ctypes.c_int32),
@@ -98,7 +104,7 @@ EXPORTS = ["amps_default_config", "amps_load_paras", "amps_create", "amps_create
            "amps_init", "amps_step", "amps_sync", "amps_monitor_now", "amps_get_field", "amps_set_field",
            "amps_get_tile_types", "amps_set_tile_types", "amps_get_monitor_values", "amps_get_info",
            "amps_set_kernel_timing", "amps_get_kernel_time", "amps_get_sweep_time",
-           "amps_tape_begin", "amps_adjoint", "amps_status_string", "amps_last_error"]
+           "amps_tape_begin", "amps_adjoint", "amps_autotune", "amps_status_string", "amps_last_error"]
 
 
 def lib():
@@ -134,6 +140,7 @@ def lib():
     L.amps_get_sweep_time.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]
     L.amps_tape_begin.argtypes = [P, ctypes.c_int]
     L.amps_adjoint.argtypes = [P, D, I32P, ctypes.c_int, D, D]
+    L.amps_autotune.argtypes = [ctypes.POINTER(Config), ctypes.c_int, ctypes.c_int, ctypes.POINTER(Tuning)]
     L.amps_status_string.argtypes = [ctypes.c_int]
     L.amps_status_string.restype = ctypes.c_char_p
     L.amps_last_error.argtypes = []
@@ -154,6 +161,24 @@ def _dptr(a):
 
 def _iptr(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def autotune(cfgs, steps):
+    """Fastest tiling (tile, wg_per_cu, us_per_step) of a run, timed on the GPU
+    (amps_autotune); apply it with tuned(cfg, tuning)."""
+    if isinstance(cfgs, Config):
+        cfgs = [cfgs]
+    arr = (Config * len(cfgs))(*[c.copy() for c in cfgs])
+    t = Tuning()
+    _check(lib().amps_autotune(arr, len(cfgs), int(steps), ctypes.byref(t)))
+    return t
+
+
+def tuned(cfg, tuning):
+    """Copy of cfg with the tiling of an amps_tuning applied."""
+    c = cfg.copy()
+    c.tile, c.wg_per_cu = tuning.tile, tuning.wg_per_cu
+    return c
 
 
 class Context:

@@ -120,6 +120,7 @@ struct StepArgs {
     const long long* step_ctr;  // device step counter (steps before this launch)
     int step_add;               // added to *step_ctr
     int ninst;                  // instances in the context (1: descriptor is loop-invariant)
+    int grid_cap;               // sweep workgroups at most (0: no cap)
     Buf in, out;
     double* aux_in;             // model auxiliary per-cell array (grain growth: sum of squares)
     double* aux_out;

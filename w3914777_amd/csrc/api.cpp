@@ -7,6 +7,8 @@
 // the explicit-Euler sweeps are all device work, replayed from hipGraphs in
 // blocks of one monitor period.
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -57,6 +59,7 @@ void set_error_text(const char* msg) { g_last_error = msg ? msg : ""; }
 struct amps_ctx {
     int model = 0, mode = 0, monitor = 0, tile = 16, every = 10, mfield = 0;
     int nfields = 0, ngroups = 1, q = 0, ninst = 0;
+    int grid_cap = 0;                      // sweep workgroups at most (wg_per_cu x CUs; 0: no cap)
     double thr_eff = 0;
     std::vector<amps_config> cfg;
     std::vector<InstDesc> desc;
@@ -143,6 +146,8 @@ int validate(const amps_config& c) {
     if (c.bc != AMPS_BC_PERIODIC && c.bc != AMPS_BC_NEUMANN) return fail(AMPS_E_ARG, "bc out of range");
     if (c.init != AMPS_INIT_REF && c.init != AMPS_INIT_CONFIG) return fail(AMPS_E_ARG, "init out of range");
     if (!finite_params(c)) return fail(AMPS_E_ARG, "non-finite parameter");
+    if (c.wg_per_cu < 0 || c.wg_per_cu > 64 || c.reserved != 0)
+        return fail(AMPS_E_ARG, "wg_per_cu must be in [0, 64] and reserved 0");
     if (c.mode == AMPS_ADAPTIVE) {
         if (c.monitor < 1 || c.monitor > 3) return fail(AMPS_E_ARG, "ADAPTIVE needs monitor 1, 2 or 3");
         if (c.monitor_every < 1) return fail(AMPS_E_ARG, "monitor_every must be >= 1");
@@ -243,6 +248,7 @@ StepArgs step_args(amps_ctx* c, int from) {
     a.step_ctr = c->d_step;
     a.step_add = 0;
     a.ninst = c->ninst;
+    a.grid_cap = c->grid_cap;
     a.in = c->buf[from];
     a.out = c->buf[from ^ 1];
     a.aux_in = c->d_aux[from];
@@ -268,7 +274,7 @@ int kev_flush(amps_ctx* c) {
 
 int launch_step_kernel(amps_ctx* c, const StepArgs& a, hipEvent_t mid = nullptr) {
     const bool adaptive = has_low(c);
-    const int grid = a.nwork;
+    const int grid = a.grid_cap > 0 && a.grid_cap < a.nwork ? a.grid_cap : a.nwork;
     if (c->model == AMPS_SNOWFLAKE) return launch_sf_step(a, c->tile, adaptive, c->stream, mid);
     int rc = c->model == AMPS_GRAINGROWTH ? launch_gg_step(a, c->tile, adaptive, grid, c->stream)
                                           : launch_sn_step(a, c->tile, adaptive, grid, c->stream);
@@ -471,6 +477,7 @@ int amps_create_batch(const amps_config* cfgs, int n, amps_ctx** out) {
         const amps_config& b = cfgs[0];
         if (a.model != b.model || a.mode != b.mode || a.tile != b.tile || a.monitor != b.monitor ||
             a.monitor_every != b.monitor_every || a.monitor_field != b.monitor_field || a.threshold != b.threshold ||
+            a.wg_per_cu != b.wg_per_cu ||
             (a.model == AMPS_GRAINGROWTH && a.q != b.q))
             return fail(AMPS_E_ARG, "batched instances must share model, mode, tile, monitor settings and q");
     }
@@ -557,6 +564,13 @@ int amps_create_batch(const amps_config* cfgs, int n, amps_ctx** out) {
     if (e != hipSuccess) { amps_destroy(c); return fail(AMPS_E_HIP, std::string("upload: ") + hipGetErrorString(e)); }
     const char* ng = std::getenv("AMPS_NO_GRAPH");
     c->use_graph = !(ng && ng[0] == '1');
+    if (cfgs[0].wg_per_cu > 0) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        c->grid_cap = cfgs[0].wg_per_cu * cus;
+    }
     c->graph_period = c->mode == AMPS_ADAPTIVE ? (c->every % 2 ? 2 * c->every : c->every) : 10;
     *out = c;
     return AMPS_OK;
@@ -955,5 +969,63 @@ extern "C" int amps_adjoint(amps_ctx* c, const double* dJ_du, const int32_t* par
             dJ_dp[(size_t)i * np + k] = sum;
         }
     }
+    return AMPS_OK;
+}
+
+// ============================================================ autotuner
+namespace {
+std::mutex g_tune_mu;
+std::map<std::string, amps_tuning> g_tune_cache;
+
+std::string tune_key(const amps_config* c, int n, int steps) {
+    std::string k = std::to_string(c[0].model) + "/" + std::to_string(c[0].mode) + "/" + std::to_string(c[0].monitor) +
+                    "/" + std::to_string(c[0].q) + "/" + std::to_string(steps) + "/" + std::to_string(n);
+    for (int i = 0; i < n; ++i) k += ":" + std::to_string(c[i].nx) + "x" + std::to_string(c[i].ny);
+    return k;
+}
+}  // namespace
+
+extern "C" int amps_autotune(const amps_config* cfgs, int n, int steps, amps_tuning* best) {
+    if (!cfgs || !best || n < 1 || n > AMPS_MAX_INST || steps < 1) return fail(AMPS_E_ARG, "bad autotune arguments");
+    const std::string key = tune_key(cfgs, n, steps);
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            *best = it->second;
+            return AMPS_OK;
+        }
+    }
+    static const int tiles[2] = {16, 32};
+    static const int dens[5] = {0, 1, 2, 3, 4};
+    amps_tuning b{0, 0, 0.0};
+    std::vector<amps_config> v(cfgs, cfgs + n);
+    const int warm = steps < 20 ? steps : 20;
+    for (int tile : tiles) {
+        bool fits = true;
+        for (int i = 0; i < n; ++i) fits = fits && cfgs[i].nx % tile == 0 && cfgs[i].ny % tile == 0;
+        if (!fits) continue;
+        for (int d : dens) {
+            for (auto& c : v) { c.tile = tile; c.wg_per_cu = d; }
+            amps_ctx* c = nullptr;
+            if (amps_create_batch(v.data(), n, &c) != AMPS_OK) continue;
+            double ms = 0.0;
+            int rc = amps_init(c);
+            if (rc == AMPS_OK) rc = amps_step(c, warm);
+            if (rc == AMPS_OK) rc = amps_sync(c, nullptr);
+            if (rc == AMPS_OK) rc = amps_step(c, steps);
+            if (rc == AMPS_OK) rc = amps_sync(c, &ms);
+            amps_destroy(c);
+            if (rc != AMPS_OK) continue;
+            const double us = 1000.0 * ms / steps;
+            if (b.tile == 0 || us < b.us_per_step) b = amps_tuning{tile, d, us};
+        }
+    }
+    if (b.tile == 0) return fail(AMPS_E_UNSUPPORTED, "no candidate tiling could run");
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        g_tune_cache[key] = b;
+    }
+    *best = b;
     return AMPS_OK;
 }

@@ -6,22 +6,306 @@
 //   lap   = (sum of the 8 Moore neighbours - 8 eta) / dxdy
 //   df    = 2 eta sum_{j!=i} eta_j^2 + eta^3 - eta          (alpha=beta=gamma=1)
 //   eta' = eta - dt L (df - kappa lap)
-// Here every field of a tile is advanced by one workgroup.  The per-cell sum
-// of squares S = sum_j eta_j^2 of the current state is carried in an FP64
-// side array written by the previous sweep, so sum_{j!=i} = S - eta_i^2 costs
-// O(1) per field instead of O(q): traffic per cell-update is 16 B per FP64
-// field (4 B per fp16 field) + 16 B for S.
+// Here every field of a tile is advanced by one wave.  The per-cell sum of
+// squares S = sum_j eta_j^2 of the current state is carried in an FP64 side
+// array written by the previous sweep, so sum_{j!=i} = S - eta_i^2 costs O(1)
+// per field instead of O(q): traffic per cell-update is 16 B per FP64 field
+// (4 B per fp16 field) + 16 B for S.
 //
 // Tile types are per (field, tile): a field that is ~0 across a tile (a grain
 // that is absent there) stays fp16 while the few fields forming the local
 // grain boundaries run in FP64.
+//
+// Work mapping (16x16 tiles): a workgroup is ONE wave and owns one tile; it
+// walks the q fields of the tile with no workgroup barrier (a one-wave
+// barrier is free), each lane updating a column strip of R = TS*TS/64 cells.
+// Per field the wave stages the (TS+2)^2 box in LDS (8 KB per wave with the
+// neighbour tables), so many independent waves per CU hide each other's load
+// latency; a low-precision field's re-centring needs only a wave reduction.
+// 32x32 tiles use a 256-thread workgroup per tile with the same arithmetic.
 #include "kernels.hpp"
 #include "tile_common.hpp"
 
 namespace amps {
 
+constexpr int GW = 64;   // one wave per workgroup
+
+// One-wave workgroup, 16x16 tile: lane l owns the row strip of 4 cells
+// (row l/4, columns 4(l%4) .. +3).  LDS rows are padded so that each lane's
+// strip and its neighbours are read with aligned 16-byte loads.
+struct GgWaveLds {
+    static constexpr int TS = 16, LWD = 20, C0D = 2, LWF = 24, C0F = 4;
+    union {
+        double d[(TS + 2) * LWD];
+        float f[(TS + 2) * LWF];
+    };
+    // precision state of the 3x3 tiles around the tile, per field
+    int ntype[AMPS_MAX_FIELDS][9];
+    double noff[AMPS_MAX_FIELDS][9];
+    float nscl[AMPS_MAX_FIELDS][9];
+};
+
+// Moore or 5-point Laplacians of a row strip of 4 cells: u, m, d hold the 6
+// values of columns x0-1 .. x0+4 of the rows above, at and below the strip;
+// operand order as the reference's (graingrowth/function.h:20).  Written
+// with constant indices so that the rows stay in registers.
+template <typename T>
+__device__ inline void gg_lap_row4(const T (&u)[6], const T (&m)[6], const T (&d)[6], bool moore, T inv, T (&lap)[4]) {
+    if (moore) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            lap[i] = (u[i] + u[i + 1] + u[i + 2] + m[i] + m[i + 2] + d[i] + d[i + 1] + d[i + 2] - T(8) * m[i + 1]) * inv;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) lap[i] = (m[i] + m[i + 2] + u[i + 1] + d[i + 1] - T(4) * m[i + 1]) * inv;
+    }
+}
+
+// Column strip form (32x32 workgroup kernel): cv[] holds rows ly0-1 .. ly0+R
+// of the column; side columns are read from LDS.
+template <typename T>
+__device__ inline T gg_lap(const T* B, int c, int LW, const T* cv, int i, bool moore, T inv) {
+    if (moore)
+        return (B[c - LW - 1] + cv[i] + B[c - LW + 1] + B[c - 1] + B[c + 1] + B[c + LW - 1] + cv[i + 2] +
+                B[c + LW + 1] - T(8) * cv[i + 1]) * inv;
+    return (B[c - 1] + B[c + 1] + cv[i] + cv[i + 2] - T(4) * cv[i + 1]) * inv;
+}
+
+// Halo element h (0..67) of a 16x16 tile: box position (by, bx) with
+// by, bx in [-1, 16], top row, bottom row, left column, right column.
+__device__ inline void gg_halo16(int h, int& by, int& bx) {
+    if (h < 18) { by = -1; bx = h - 1; }
+    else if (h < 36) { by = 16; bx = h - 19; }
+    else if (h < 52) { by = h - 36; bx = -1; }
+    else { by = h - 52; bx = 16; }
+}
+
+// Field data of one field for the lane: own strip (4 FP64 values or 4 fp16
+// words) and up to two halo elements (FP64 bits or an fp16 word).
+struct GgRaw {
+    double own[4];
+    uint2 own16;
+    unsigned long long halo[2];
+};
+
+template <bool ADAPT>
+__device__ inline void gg_wave_load(GgRaw& r, const StepArgs& a, const InstDesc& d, int f, int ty, int cell0,
+                                    const int* hoff, const int* hreg, int nh, const GgWaveLds& L) {
+    const double* f64 = a.in.f64 + (size_t)f * a.total_cells + d.cell_base;
+    const unsigned short* f16 =
+        reinterpret_cast<const unsigned short*>(a.in.f16 + (size_t)f * a.total_cells + d.cell_base);
+    if (ty == 2) {
+        const double2 u = *reinterpret_cast<const double2*>(f64 + cell0);
+        const double2 v = *reinterpret_cast<const double2*>(f64 + cell0 + 2);
+        r.own[0] = u.x; r.own[1] = u.y; r.own[2] = v.x; r.own[3] = v.y;
+    } else {
+        r.own16 = *reinterpret_cast<const uint2*>(f16 + cell0);
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        if (k < nh) {
+            if (!ADAPT || L.ntype[f][hreg[k]] == 2) r.halo[k] = (unsigned long long)__double_as_longlong(f64[hoff[k]]);
+            else r.halo[k] = f16[hoff[k]];
+        }
+    }
+}
+
+__device__ inline float half_word(unsigned w) { return __half2float(__ushort_as_half((unsigned short)w)); }
+
+// Read-only metadata (descriptors, tile -> instance map, tile types) arrive
+// as restrict-qualified arguments so that the per-tile reads go through the
+// scalar cache; with MULTI = false the descriptor is loop-invariant.
+// One-wave mapping of 16x16 tiles: per field one vector load of the lane's
+// own strip and at most two halo elements (offsets computed once per tile),
+// the next field's loads in flight while the current field is computed.
+template <int TS, bool ADAPT, bool MULTI>
+__global__ __launch_bounds__(64) void gg_wave_kernel(StepArgs a, const InstDesc* __restrict__ inst,
+                                                     const int* __restrict__ tinst, const int* __restrict__ types) {
+    static_assert(TS == 16, "the one-wave mapping covers 16x16 tiles");
+    using Lw = GgWaveLds;
+    constexpr int LWD = Lw::LWD, C0D = Lw::C0D, LWF = Lw::LWF, C0F = Lw::C0F;
+    __shared__ __align__(16) GgWaveLds L;
+    const int lane = threadIdx.x;
+    const int ly = lane >> 2, x0 = (lane & 3) << 2;
+    const int nh = lane + 64 < 68 ? 2 : 1;
+    for (int b = blockIdx.x; b < a.nwork; b += gridDim.x) {
+        const int w = xcd_remap(b, a.nwork);
+        const InstDesc& d = inst[MULTI ? tinst[w] : 0];
+        const int lt = w - d.tile_base;
+        const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
+        const int ox = tile_x * TS, oy = tile_y * TS;
+        const bool interior = ox >= 1 && oy >= 1 && ox + TS + 1 <= d.nx && oy + TS + 1 <= d.ny;
+        const double inv = d.k[GGK_INV_DXDY], dtl = d.k[GGK_DT_L], kap = d.k[GGK_KAPPA];
+        const double al = d.k[GGK_ALPHA], be = d.k[GGK_BETA], g2 = d.k[GGK_2GAMMA];
+        const bool moore = d.stencil == AMPS_STENCIL_REF;
+        const int cell0 = (oy + ly) * d.nx + ox + x0;
+        // halo elements of the lane: cell offsets, regions, LDS rows/columns
+        int hoff[2] = {0, 0}, hreg[2] = {4, 4}, hby[2] = {0, 0}, hbx[2] = {0, 0};
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if (k < nh) {
+                int by, bx;
+                gg_halo16(lane + 64 * k, by, bx);
+                int gy = oy + by, gx = ox + bx;
+                if (!interior) { gy = bmap(gy, d.ny, d.bc); gx = bmap(gx, d.nx, d.bc); }
+                hoff[k] = gy * d.nx + gx;
+                hreg[k] = (by < 0 ? 0 : (by >= TS ? 6 : 3)) + (bx < 0 ? 0 : (bx >= TS ? 2 : 1));
+                hby[k] = by + 1;
+                hbx[k] = bx;
+            }
+        }
+        if (ADAPT) {
+            Nb nb;
+            nb_setup(nb, d, tile_x, tile_y, TS);
+            for (int e = lane; e < a.q * 9; e += GW) {
+                const int f = e / 9, k = e - f * 9;
+                const int g = nb_tile(nb, k);
+                L.ntype[f][k] = types[(size_t)f * a.total_tiles + g];
+                L.noff[f][k] = a.in.off[(size_t)f * a.total_tiles + g];
+                L.nscl[f][k] = a.in.scl[(size_t)f * a.total_tiles + g];
+            }
+        }
+        double S[4], Sn[4] = {0.0, 0.0, 0.0, 0.0};
+        {
+            const double2 u = *reinterpret_cast<const double2*>(a.aux_in + d.cell_base + cell0);
+            const double2 v = *reinterpret_cast<const double2*>(a.aux_in + d.cell_base + cell0 + 2);
+            S[0] = u.x; S[1] = u.y; S[2] = v.x; S[3] = v.y;
+        }
+        __syncthreads();  // neighbour tables visible
+        GgRaw raw;
+        gg_wave_load<ADAPT>(raw, a, d, 0, ADAPT ? L.ntype[0][4] : 2, cell0, hoff, hreg, nh, L);
+        for (int f = 0; f < a.q; ++f) {
+            const int ty = ADAPT ? L.ntype[f][4] : 2;
+            // stage field f (own strip + halo) into the LDS box
+            if (ty == 2) {
+                double* row = L.d + (ly + 1) * LWD + C0D + x0;
+                *reinterpret_cast<double2*>(row) = make_double2(raw.own[0], raw.own[1]);
+                *reinterpret_cast<double2*>(row + 2) = make_double2(raw.own[2], raw.own[3]);
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    if (k < nh) {
+                        const int r = hreg[k];
+                        L.d[hby[k] * LWD + C0D + hbx[k]] =
+                            (!ADAPT || L.ntype[f][r] == 2)
+                                ? __longlong_as_double((long long)raw.halo[k])
+                                : L.noff[f][r] + (double)half_word((unsigned)raw.halo[k]) * (double)L.nscl[f][r];
+                    }
+                }
+            } else {
+                const double own_off = L.noff[f][4];
+                const float sc = L.nscl[f][4];
+                float* row = L.f + (ly + 1) * LWF + C0F + x0;
+                *reinterpret_cast<float4*>(row) =
+                    make_float4(half_word(raw.own16.x & 0xFFFFu) * sc, half_word(raw.own16.x >> 16) * sc,
+                                half_word(raw.own16.y & 0xFFFFu) * sc, half_word(raw.own16.y >> 16) * sc);
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    if (k < nh) {
+                        const int r = hreg[k];
+                        float x;
+                        if (L.ntype[f][r] == 2) x = (float)(__longlong_as_double((long long)raw.halo[k]) - own_off);
+                        else x = (float)(L.noff[f][r] - own_off) + half_word((unsigned)raw.halo[k]) * L.nscl[f][r];
+                        L.f[hby[k] * LWF + C0F + hbx[k]] = x;
+                    }
+                }
+            }
+            if (f + 1 < a.q) gg_wave_load<ADAPT>(raw, a, d, f + 1, ADAPT ? L.ntype[f + 1][4] : 2, cell0, hoff, hreg, nh, L);
+            __syncthreads();
+            if (ty == 2) {
+                double ru[6], rm[6], rd[6];
+                {
+                    const double* p = L.d + ly * LWD + C0D + x0 - 1;
+#define GG_ROW6D(R, P)                                                                   \
+    do {                                                                                 \
+        R[0] = (P)[0];                                                                   \
+        const double2 m1_ = *reinterpret_cast<const double2*>((P) + 1);                  \
+        const double2 m2_ = *reinterpret_cast<const double2*>((P) + 3);                  \
+        R[1] = m1_.x; R[2] = m1_.y; R[3] = m2_.x; R[4] = m2_.y;                          \
+        R[5] = (P)[5];                                                                   \
+    } while (0)
+                    GG_ROW6D(ru, p);
+                    GG_ROW6D(rm, p + LWD);
+                    GG_ROW6D(rd, p + 2 * LWD);
+#undef GG_ROW6D
+                }
+                double lapv[4], en[4];
+                gg_lap_row4<double>(ru, rm, rd, moore, inv, lapv);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const double eta = rm[i + 1];
+                    const double lap = lapv[i];
+                    const double others = S[i] - eta * eta;
+                    const double df = g2 * eta * others + be * (eta * eta * eta) - al * eta;
+                    en[i] = eta - dtl * (df - kap * lap);
+                    Sn[i] += en[i] * en[i];
+                }
+                double* out = a.out.f64 + (size_t)f * a.total_cells + d.cell_base + cell0;
+                *reinterpret_cast<double2*>(out) = make_double2(en[0], en[1]);
+                *reinterpret_cast<double2*>(out + 2) = make_double2(en[2], en[3]);
+            } else {
+                const float invf = d.kf[GGK_INV_DXDY], dtlf = d.kf[GGK_DT_L], kapf = d.kf[GGK_KAPPA];
+                const float alf = d.kf[GGK_ALPHA], bef = d.kf[GGK_BETA], g2f = d.kf[GGK_2GAMMA];
+                const double own_off = L.noff[f][4];
+                const float offf = (float)own_off;
+                float ru[6], rm[6], rd[6];
+                {
+                    const float* p = L.f + ly * LWF + C0F + x0;
+#define GG_ROW6F(R, P)                                                                   \
+    do {                                                                                 \
+        R[0] = (P)[-1];                                                                  \
+        const float4 m_ = *reinterpret_cast<const float4*>(P);                           \
+        R[1] = m_.x; R[2] = m_.y; R[3] = m_.z; R[4] = m_.w;                              \
+        R[5] = (P)[4];                                                                   \
+    } while (0)
+                    GG_ROW6F(ru, p);
+                    GG_ROW6F(rm, p + LWF);
+                    GG_ROW6F(rd, p + 2 * LWF);
+#undef GG_ROW6F
+                }
+                float lapv[4], rn[4];
+                gg_lap_row4<float>(ru, rm, rd, moore, invf, lapv);
+                float mn = INFINITY, mx = -INFINITY;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float dv = rm[i + 1];
+                    const float lap = lapv[i];
+                    const float eta = offf + dv;
+                    const float others = (float)S[i] - eta * eta;
+                    const float df = g2f * eta * others + bef * (eta * eta * eta) - alf * eta;
+                    rn[i] = dv - dtlf * (df - kapf * lap);
+                    mn = fminf(mn, rn[i]);
+                    mx = fmaxf(mx, rn[i]);
+                }
+                mn = wave_min(mn);
+                mx = wave_max(mx);
+                const TileCode tc = tile_code(mn, mx);
+                const double noff = own_off + (double)tc.mid;
+                unsigned short hw[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const __half h = __float2half_rn((rn[i] - tc.mid) * tc.inv);
+                    hw[i] = __half_as_ushort(h);
+                    const double v = noff + (double)__half2float(h) * (double)tc.scl;
+                    Sn[i] += v * v;
+                }
+                __half* out = a.out.f16 + (size_t)f * a.total_cells + d.cell_base;
+                *reinterpret_cast<uint2*>(out + cell0) =
+                    make_uint2((unsigned)hw[0] | ((unsigned)hw[1] << 16), (unsigned)hw[2] | ((unsigned)hw[3] << 16));
+                if (lane == 0) {
+                    a.out.off[(size_t)f * a.total_tiles + w] = noff;
+                    a.out.scl[(size_t)f * a.total_tiles + w] = tc.scl;
+                }
+            }
+            __syncthreads();  // the box is rewritten for the next field
+        }
+        double* so = a.aux_out + d.cell_base + cell0;
+        *reinterpret_cast<double2*>(so) = make_double2(Sn[0], Sn[1]);
+        *reinterpret_cast<double2*>(so + 2) = make_double2(Sn[2], Sn[3]);
+    }
+}
+
 template <int TS>
-struct GgLds {
+struct GgBlockLds {
     static constexpr int H = 1, LW = TS + 2;
     union {
         double d[LW * LW];
@@ -34,21 +318,11 @@ struct GgLds {
     float red[8];
 };
 
-// Moore or 5-point Laplacian of the cell at strip position i (column values
-// cv[] hold rows ly0-1 .. ly0+R of the column; side columns read from LDS).
-template <typename T>
-__device__ inline T gg_lap(const T* B, int c, int LW, const T* cv, int i, bool moore, T inv) {
-    if (moore)
-        return (B[c - LW - 1] + cv[i] + B[c - LW + 1] + B[c - 1] + B[c + 1] + B[c + LW - 1] + cv[i + 2] +
-                B[c + LW + 1] - T(8) * cv[i + 1]) * inv;
-    return (B[c - 1] + B[c + 1] + cv[i] + cv[i + 2] - T(4) * cv[i + 1]) * inv;
-}
-
 // Raw box values of field f: FP64 bits, or the fp16 word where the element's
 // tile is low (the type table in LDS says which storage is current).
 template <int TS, bool ADAPT>
 __device__ inline void gg_load_raw(unsigned long long* raw, const StepArgs& a, const InstDesc& d, int f,
-                                   const BoxMap<TS, 1>& m, const GgLds<TS>& L) {
+                                   const BoxMap<TS, 1>& m, const GgBlockLds<TS>& L) {
     constexpr int N = BoxMap<TS, 1>::N;
     const double* f64 = a.in.f64 + (size_t)f * a.total_cells + d.cell_base;
     const unsigned short* f16 =
@@ -62,19 +336,16 @@ __device__ inline void gg_load_raw(unsigned long long* raw, const StepArgs& a, c
     }
 }
 
-// Read-only metadata (descriptors, tile -> instance map, tile types) arrive
-// as restrict-qualified arguments so that the per-tile reads go through the
-// scalar cache; with MULTI = false the descriptor is loop-invariant.
-//
-// Per tile: one memory round brings the neighbour precision table of every
+// Workgroup-per-tile variant for 32x32 tiles (a one-wave mapping of a 32x32
+// tile needs more registers than a SIMD can give 4 waves).  Per tile: one memory round brings the neighbour precision table of every
 // field, the carried sum of squares and field 0's box; afterwards the box of
 // field f+1 is loaded into registers while field f is computed, so each field
 // costs one overlapped load round.
 template <int TS, bool ADAPT, bool MULTI>
-__global__ __launch_bounds__(256, 4) void gg_step_kernel(StepArgs a, const InstDesc* __restrict__ inst,
+__global__ __launch_bounds__(256, 4) void gg_block_kernel(StepArgs a, const InstDesc* __restrict__ inst,
                                                       const int* __restrict__ tinst, const int* __restrict__ types) {
     constexpr int H = 1, LW = TS + 2, R = TS * TS / WG, N = BoxMap<TS, 1>::N;
-    __shared__ GgLds<TS> L;
+    __shared__ GgBlockLds<TS> L;
     for (int b = blockIdx.x; b < a.nwork; b += gridDim.x) {
         const int w = xcd_remap(b, a.nwork);
         const InstDesc& d = inst[MULTI ? tinst[w] : 0];
@@ -244,10 +515,17 @@ __global__ __launch_bounds__(256) void gg_aux_kernel(StepArgs a) {
 
 template <int TS, bool ADAPT>
 static void gg_launch(const StepArgs& a, int grid, hipStream_t s) {
-    if (a.ninst > 1)
-        hipLaunchKernelGGL((gg_step_kernel<TS, ADAPT, true>), dim3(grid), dim3(WG), 0, s, a, a.inst, a.tile_inst, a.type);
-    else
-        hipLaunchKernelGGL((gg_step_kernel<TS, ADAPT, false>), dim3(grid), dim3(WG), 0, s, a, a.inst, a.tile_inst, a.type);
+    if constexpr (TS == 16) {
+        if (a.ninst > 1)
+            hipLaunchKernelGGL((gg_wave_kernel<TS, ADAPT, true>), dim3(grid), dim3(GW), 0, s, a, a.inst, a.tile_inst, a.type);
+        else
+            hipLaunchKernelGGL((gg_wave_kernel<TS, ADAPT, false>), dim3(grid), dim3(GW), 0, s, a, a.inst, a.tile_inst, a.type);
+    } else {
+        if (a.ninst > 1)
+            hipLaunchKernelGGL((gg_block_kernel<TS, ADAPT, true>), dim3(grid), dim3(WG), 0, s, a, a.inst, a.tile_inst, a.type);
+        else
+            hipLaunchKernelGGL((gg_block_kernel<TS, ADAPT, false>), dim3(grid), dim3(WG), 0, s, a, a.inst, a.tile_inst, a.type);
+    }
 }
 
 int launch_gg_step(const StepArgs& a, int tile, bool adaptive, int grid, hipStream_t s) {

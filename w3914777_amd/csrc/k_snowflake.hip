@@ -622,7 +622,7 @@ __global__ __launch_bounds__(256, 5) void sf_low_kernel(StepArgs a, const InstDe
 // Persistent grid of kernel K: resident workgroups per CU (occupancy query,
 // cached per kernel) x CUs, at most one workgroup per work item.
 template <auto K>
-static int resident_grid(int nwork) {
+static int resident_grid(int nwork, int cap) {
     static int per_cu = 0, cus = 0;
     if (per_cu == 0) {
         int v = 0, dev = 0, c = 0;
@@ -633,7 +633,8 @@ static int resident_grid(int nwork) {
         cus = c;
         per_cu = v;
     }
-    const long long g = (long long)per_cu * cus;
+    long long g = (long long)per_cu * cus;
+    if (cap > 0 && cap < g) g = cap;
     return (int)(g < nwork ? g : nwork);
 }
 
@@ -643,14 +644,14 @@ static void sf_launch(const StepArgs& a, bool adaptive, hipStream_t s, hipEvent_
     if (adaptive) {
         constexpr auto kh = sf_high_kernel<TS, true, MULTI>;
         constexpr auto kl = sf_low_kernel<TS, MULTI>;
-        hipLaunchKernelGGL(kh, dim3(resident_grid<kh>(a.total_tiles)), blk, 0, s, a, a.inst, a.tile_inst, a.worklist,
+        hipLaunchKernelGGL(kh, dim3(resident_grid<kh>(a.total_tiles, a.grid_cap)), blk, 0, s, a, a.inst, a.tile_inst, a.worklist,
                            a.wl_count);
         if (mid) (void)hipEventRecord(mid, s);
-        hipLaunchKernelGGL(kl, dim3(resident_grid<kl>(a.total_tiles)), blk, 0, s, a, a.inst, a.tile_inst, a.wl_low,
+        hipLaunchKernelGGL(kl, dim3(resident_grid<kl>(a.total_tiles, a.grid_cap)), blk, 0, s, a, a.inst, a.tile_inst, a.wl_low,
                            a.wl_low_count);
     } else {
         constexpr auto kh = sf_high_kernel<TS, false, MULTI>;
-        hipLaunchKernelGGL(kh, dim3(resident_grid<kh>(a.nwork)), blk, 0, s, a, a.inst, a.tile_inst,
+        hipLaunchKernelGGL(kh, dim3(resident_grid<kh>(a.nwork, a.grid_cap)), blk, 0, s, a, a.inst, a.tile_inst,
                            (const int*)nullptr, (const int*)nullptr);
         if (mid) (void)hipEventRecord(mid, s);
     }
