@@ -80,16 +80,24 @@ def run_path(amps, cfg, warmup, steps, kernel_pass=True, with_fields=True):
 
 
 def copy_probe():
-    """A plain device copy of known size (1 GiB read + 1 GiB write), the
-    calibration reference of the FETCH_SIZE / WRITE_SIZE counters."""
-    import torch
-    n = 1 << 27
-    a = torch.ones(n, dtype=torch.float64, device="cuda")
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
-    b.copy_(a)
-    torch.cuda.synchronize()
-    del a, b
+    """A plain device-to-device copy of known size (1 GiB read + 1 GiB write,
+    the HIP runtime's blit kernel), the calibration reference of the
+    FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = ctypes.c_size_t(1 << 30)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), n) != 0 or hip.hipMalloc(ctypes.byref(b), n) != 0:
+        raise RuntimeError("copy probe: hipMalloc failed")
+    try:
+        hip.hipMemset(a, 1, n)
+        hip.hipDeviceSynchronize()
+        if hip.hipMemcpy(b, a, n, 3) != 0:   # hipMemcpyDeviceToDevice
+            raise RuntimeError("copy probe: hipMemcpy failed")
+        hip.hipDeviceSynchronize()
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
 
 
 def traffic_from_profiles():

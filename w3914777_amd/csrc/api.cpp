@@ -300,10 +300,10 @@ int enqueue_step(amps_ctx* c, int from, int step_add, bool capturing = false) {
 
 // Model auxiliary state that must follow the fields (grain growth: the
 // per-cell sum of squares, recomputed from current storage).
-int refresh_aux(amps_ctx* c, int b) {
+int refresh_aux(amps_ctx* c, int b, bool changed_only = false) {
     if (c->model != AMPS_GRAINGROWTH) return AMPS_OK;
     StepArgs a = step_args(c, b);
-    return launch_gg_aux(a, c->tile, c->stream);
+    return launch_gg_aux(a, c->tile, changed_only && a.type ? c->d_type_old : nullptr, c->stream);
 }
 
 // Rebuild the FP64 / low tile work-lists from the current types (snowflake).
@@ -328,7 +328,7 @@ int enqueue_monitor(amps_ctx* c, int b) {
                           c->d_type_old, c->d_type, c->d_stats, c->stream));
     RCCHK(launch_convert(c->d_desc, c->d_tile_inst, c->buf[b], c->total_cells, c->total_tiles, c->d_type_old,
                          c->d_type, c->gm, CONVERT_CHANGED, c->tile, c->stream));
-    if (c->model == AMPS_GRAINGROWTH) RCCHK(refresh_aux(c, b));
+    if (c->model == AMPS_GRAINGROWTH) RCCHK(refresh_aux(c, b, true));
     RCCHK(refresh_worklists(c));
     return AMPS_OK;
 }

@@ -287,6 +287,118 @@ __global__ __launch_bounds__(256) void monitor_grad_kernel(StepArgs a, int ngrou
     }
 }
 
+// The same monitor for 16x16 tiles with one wave per tile evaluating every
+// monitored field: lane l owns the row strip (row l/4, columns 4(l%4)..+3),
+// loaded with one vector load, plus at most two halo elements whose offsets
+// are computed once per tile; the neighbour precision table of every field
+// is read once.  Values are decoded and the quotient evaluated exactly as
+// above, so the monitor values are identical.
+struct MonWaveLds {
+    double box[18 * 20];
+    int ntype[AMPS_MAX_FIELDS][9];
+    double noff[AMPS_MAX_FIELDS][9];
+    float nscl[AMPS_MAX_FIELDS][9];
+};
+
+__global__ __launch_bounds__(64) void monitor_grad_wave16(StepArgs a, const InstDesc* __restrict__ inst,
+                                                          const int* __restrict__ tinst,
+                                                          const int* __restrict__ types, int ngroups_eval, int field0,
+                                                          int field_stride, int type_groups, double* __restrict__ md) {
+    constexpr int TS = 16, LWD = 20, C0 = 2;
+    __shared__ __align__(16) MonWaveLds L;
+    const int lane = threadIdx.x;
+    const int ly = lane >> 2, x0 = (lane & 3) << 2;
+    const int nh = lane + 64 < 68 ? 2 : 1;
+    const bool uni = types == nullptr;
+    for (int w = blockIdx.x; w < a.total_tiles; w += gridDim.x) {
+        const InstDesc& d = inst[tinst[w]];
+        const int lt = w - d.tile_base, tyi = lt / d.tx, txi = lt - tyi * d.tx;
+        const int ox = txi * TS, oy = tyi * TS;
+        const bool interior = ox >= 1 && oy >= 1 && ox + TS + 1 <= d.nx && oy + TS + 1 <= d.ny;
+        const int cell0 = (oy + ly) * d.nx + ox + x0;
+        int hoff[2] = {0, 0}, hreg[2] = {4, 4}, hidx[2] = {0, 0};
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if (k < nh) {
+                const int h = lane + 64 * k;
+                int by, bx;
+                if (h < 18) { by = -1; bx = h - 1; }
+                else if (h < 36) { by = 16; bx = h - 19; }
+                else if (h < 52) { by = h - 36; bx = -1; }
+                else { by = h - 52; bx = 16; }
+                int gy = oy + by, gx = ox + bx;
+                if (!interior) { gy = bmap(gy, d.ny, d.bc); gx = bmap(gx, d.nx, d.bc); }
+                hoff[k] = gy * d.nx + gx;
+                hreg[k] = (by < 0 ? 0 : (by >= TS ? 6 : 3)) + (bx < 0 ? 0 : (bx >= TS ? 2 : 1));
+                hidx[k] = (by + 1) * LWD + C0 + bx;
+            }
+        }
+        if (!uni) {
+            Nb nb;
+            nb_setup(nb, d, txi, tyi, TS);
+            for (int e = lane; e < ngroups_eval * 9; e += 64) {
+                const int g = e / 9, k = e - g * 9;
+                const int field = field0 + g * field_stride, tgroup = type_groups > 1 ? g : 0;
+                const int gt = nb_tile(nb, k);
+                L.ntype[g][k] = types[(size_t)tgroup * a.total_tiles + gt];
+                L.noff[g][k] = a.in.off[(size_t)field * a.total_tiles + gt];
+                L.nscl[g][k] = a.in.scl[(size_t)field * a.total_tiles + gt];
+            }
+        }
+        __syncthreads();
+        const double r = d.p[0] / d.p[1];  // dx / dy (slots 0 and 1 of every model)
+        for (int g = 0; g < ngroups_eval; ++g) {
+            const int field = field0 + g * field_stride;
+            const double* f64 = a.in.f64 + (size_t)field * a.total_cells + d.cell_base;
+            const __half* f16 = a.in.f16 + (size_t)field * a.total_cells + d.cell_base;
+            double own[4];
+            if (uni || L.ntype[g][4] == 2) {
+                const double2 u = *reinterpret_cast<const double2*>(f64 + cell0);
+                const double2 v = *reinterpret_cast<const double2*>(f64 + cell0 + 2);
+                own[0] = u.x; own[1] = u.y; own[2] = v.x; own[3] = v.y;
+            } else {
+                const uint2 hw = *reinterpret_cast<const uint2*>(f16 + cell0);
+                const double o = L.noff[g][4], sc = (double)L.nscl[g][4];
+                own[0] = o + (double)__half2float(__ushort_as_half((unsigned short)(hw.x & 0xFFFFu))) * sc;
+                own[1] = o + (double)__half2float(__ushort_as_half((unsigned short)(hw.x >> 16))) * sc;
+                own[2] = o + (double)__half2float(__ushort_as_half((unsigned short)(hw.y & 0xFFFFu))) * sc;
+                own[3] = o + (double)__half2float(__ushort_as_half((unsigned short)(hw.y >> 16))) * sc;
+            }
+            double hv[2] = {0.0, 0.0};
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                if (k < nh) {
+                    const int rg = hreg[k];
+                    if (uni || L.ntype[g][rg] == 2) hv[k] = f64[hoff[k]];
+                    else hv[k] = L.noff[g][rg] + (double)__half2float(f16[hoff[k]]) * (double)L.nscl[g][rg];
+                }
+            }
+            double* row = L.box + (ly + 1) * LWD + C0 + x0;
+            *reinterpret_cast<double2*>(row) = make_double2(own[0], own[1]);
+            *reinterpret_cast<double2*>(row + 2) = make_double2(own[2], own[3]);
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+                if (k < nh) L.box[hidx[k]] = hv[k];
+            __syncthreads();
+            double m = 0.0;
+            {
+#pragma clang fp contract(off)
+                const double* c = L.box + (ly + 1) * LWD + C0 + x0;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const double hx = 0.5 * (c[i + 1] - c[i - 1]);
+                    const double hy = 0.5 * (c[i + LWD] - c[i - LWD]) * r;
+                    const double qv = hx * hx + hy * hy;
+                    if (qv > m) m = qv;
+                }
+            }
+            m = wave_max_d(m);
+            if (lane == 0) md[(size_t)g * a.total_tiles + w] = m;
+            __syncthreads();
+        }
+    }
+}
+
 // get_type (snowflake/tools.h:212-230): type_old <- type; neighbour max over
 // self and the 4 non-periodic von-Neumann neighbours with fmax (CUDA's max);
 // < thr -> 1, >= thr -> 2, unordered keeps the old type.  One thread per
@@ -419,7 +531,11 @@ int launch_monitor(const StepArgs& a, int kind, int tile, int ngroups_eval, int 
     const int n = ngroups_eval * a.total_tiles;
     if (kind == AMPS_MON_GRADIENT) {
         int grid = n > 65536 ? 65536 : n;
-        if (tile == 16) hipLaunchKernelGGL((monitor_grad_kernel<16>), dim3(grid), dim3(256), 0, s, a, ngroups_eval, field0, field_stride, type_groups, md);
+        if (tile == 16) {
+            const int g16 = a.total_tiles > 262144 ? 262144 : a.total_tiles;
+            hipLaunchKernelGGL(monitor_grad_wave16, dim3(g16), dim3(64), 0, s, a, a.inst, a.tile_inst, a.type,
+                               ngroups_eval, field0, field_stride, type_groups, md);
+        }
         else if (tile == 32) hipLaunchKernelGGL((monitor_grad_kernel<32>), dim3(grid), dim3(256), 0, s, a, ngroups_eval, field0, field_stride, type_groups, md);
         else if (tile == 64) hipLaunchKernelGGL((monitor_grad_kernel<64>), dim3(grid), dim3(256), 0, s, a, ngroups_eval, field0, field_stride, type_groups, md);
         else return AMPS_E_UNSUPPORTED;

@@ -475,11 +475,20 @@ __global__ __launch_bounds__(256, 4) void gg_block_kernel(StepArgs a, const Inst
 }
 
 // S = sum_i eta_i^2 from the current storage of buffer `in` (low tiles
-// decoded), written to aux_in.  One workgroup per tile.
+// decoded), written to aux_in.  One workgroup per tile.  With type_old given,
+// only tiles where some field changed precision are refreshed (a conversion
+// re-rounds the stored values; elsewhere S already matches them).
 template <int TS>
-__global__ __launch_bounds__(256) void gg_aux_kernel(StepArgs a) {
+__global__ __launch_bounds__(256) void gg_aux_kernel(StepArgs a, const int* __restrict__ type_old,
+                                                     const int* __restrict__ types) {
     constexpr int PER = TS * TS / WG;
     for (int w = blockIdx.x; w < a.total_tiles; w += gridDim.x) {
+        if (type_old) {
+            bool changed = false;
+            for (int f = 0; f < a.q; ++f)
+                changed = changed || type_old[(size_t)f * a.total_tiles + w] != types[(size_t)f * a.total_tiles + w];
+            if (!changed) continue;
+        }
         const InstDesc& d = a.inst[a.tile_inst[w]];
         const int lt = w - d.tile_base;
         const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
@@ -541,10 +550,10 @@ int launch_gg_step(const StepArgs& a, int tile, bool adaptive, int grid, hipStre
     return hipGetLastError() == hipSuccess ? AMPS_OK : AMPS_E_HIP;
 }
 
-int launch_gg_aux(const StepArgs& a, int tile, hipStream_t s) {
+int launch_gg_aux(const StepArgs& a, int tile, const int* type_old, hipStream_t s) {
     int grid = a.total_tiles > 65536 ? 65536 : a.total_tiles;
-    if (tile == 16) hipLaunchKernelGGL((gg_aux_kernel<16>), dim3(grid), dim3(WG), 0, s, a);
-    else if (tile == 32) hipLaunchKernelGGL((gg_aux_kernel<32>), dim3(grid), dim3(WG), 0, s, a);
+    if (tile == 16) hipLaunchKernelGGL((gg_aux_kernel<16>), dim3(grid), dim3(WG), 0, s, a, type_old, a.type);
+    else if (tile == 32) hipLaunchKernelGGL((gg_aux_kernel<32>), dim3(grid), dim3(WG), 0, s, a, type_old, a.type);
     else return AMPS_E_UNSUPPORTED;
     return hipGetLastError() == hipSuccess ? AMPS_OK : AMPS_E_HIP;
 }
