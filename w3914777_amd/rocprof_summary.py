@@ -51,10 +51,10 @@ def main(argv=None):
                 print(f"{k[:58]:<58} {len(v):>8} {sum(v) / 1e6:>10.2f} {sum(v) / len(v) / 1e3:>10.2f} "
                       f"{min(v) / 1e3:>9.2f} {max(v) / 1e3:>9.2f}")
     if counters:
-        print(f"{'kernel':<58} {'counter':<22} {'dispatches':>10} {'mean per dispatch':>18}")
+        print(f"{'kernel':<58} {'counter':<22} {'dispatches':>10} {'mean':>14} {'max':>14} {'sum':>14}")
         for (k, c), v in sorted(counters.items()):
             if keep(k):
-                print(f"{k[:58]:<58} {c:<22} {len(v):>10} {sum(v) / len(v):>18.6g}")
+                print(f"{k[:58]:<58} {c:<22} {len(v):>10} {sum(v) / len(v):>14.6g} {max(v):>14.6g} {sum(v):>14.6g}")
     return 0
 
 
