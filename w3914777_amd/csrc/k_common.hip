@@ -136,9 +136,11 @@ __device__ void decode_tile(const Buf& b, long long total_cells, int total_tiles
 // force_decode convert every tile whose new type is 1 / every type-1 tile.
 // One workgroup per (group, tile).
 template <int TS>
-__global__ __launch_bounds__(256) void convert_kernel(const InstDesc* inst, const int* tile_inst, Buf b,
-                                                       long long total_cells, int total_tiles, const int* type_old,
-                                                       const int* type, GroupMap gm, int mode) {
+__global__ __launch_bounds__(256) void convert_kernel(const InstDesc* __restrict__ inst,
+                                                       const int* __restrict__ tile_inst, Buf b,
+                                                       long long total_cells, int total_tiles,
+                                                       const int* __restrict__ type_old,
+                                                       const int* __restrict__ type, GroupMap gm, int mode) {
     __shared__ double red[4];
     const int ngw = gm.ngroups * total_tiles;
     for (int w = blockIdx.x; w < ngw; w += gridDim.x) {
@@ -347,30 +349,52 @@ __global__ __launch_bounds__(64) void monitor_grad_wave16(StepArgs a, const Inst
         }
         __syncthreads();
         const double r = d.p[0] / d.p[1];  // dx / dy (slots 0 and 1 of every model)
-        for (int g = 0; g < ngroups_eval; ++g) {
+        // raw loads of field g: FP64 strip or fp16 words, halo FP64 bits or fp16 word
+        double ownd[4];
+        uint2 ownh = make_uint2(0u, 0u);
+        unsigned long long hraw[2] = {0ull, 0ull};
+        auto load = [&](int g) {
             const int field = field0 + g * field_stride;
             const double* f64 = a.in.f64 + (size_t)field * a.total_cells + d.cell_base;
-            const __half* f16 = a.in.f16 + (size_t)field * a.total_cells + d.cell_base;
-            double own[4];
+            const unsigned short* f16 =
+                reinterpret_cast<const unsigned short*>(a.in.f16 + (size_t)field * a.total_cells + d.cell_base);
             if (uni || L.ntype[g][4] == 2) {
                 const double2 u = *reinterpret_cast<const double2*>(f64 + cell0);
                 const double2 v = *reinterpret_cast<const double2*>(f64 + cell0 + 2);
-                own[0] = u.x; own[1] = u.y; own[2] = v.x; own[3] = v.y;
+                ownd[0] = u.x; ownd[1] = u.y; ownd[2] = v.x; ownd[3] = v.y;
             } else {
-                const uint2 hw = *reinterpret_cast<const uint2*>(f16 + cell0);
+                ownh = *reinterpret_cast<const uint2*>(f16 + cell0);
+            }
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                if (k < nh) {
+                    if (uni || L.ntype[g][hreg[k]] == 2)
+                        hraw[k] = (unsigned long long)__double_as_longlong(f64[hoff[k]]);
+                    else hraw[k] = f16[hoff[k]];
+                }
+            }
+        };
+        if (ngroups_eval > 0) load(0);
+        for (int g = 0; g < ngroups_eval; ++g) {
+            double own[4];
+            if (uni || L.ntype[g][4] == 2) {
+                own[0] = ownd[0]; own[1] = ownd[1]; own[2] = ownd[2]; own[3] = ownd[3];
+            } else {
                 const double o = L.noff[g][4], sc = (double)L.nscl[g][4];
-                own[0] = o + (double)__half2float(__ushort_as_half((unsigned short)(hw.x & 0xFFFFu))) * sc;
-                own[1] = o + (double)__half2float(__ushort_as_half((unsigned short)(hw.x >> 16))) * sc;
-                own[2] = o + (double)__half2float(__ushort_as_half((unsigned short)(hw.y & 0xFFFFu))) * sc;
-                own[3] = o + (double)__half2float(__ushort_as_half((unsigned short)(hw.y >> 16))) * sc;
+                own[0] = o + (double)__half2float(__ushort_as_half((unsigned short)(ownh.x & 0xFFFFu))) * sc;
+                own[1] = o + (double)__half2float(__ushort_as_half((unsigned short)(ownh.x >> 16))) * sc;
+                own[2] = o + (double)__half2float(__ushort_as_half((unsigned short)(ownh.y & 0xFFFFu))) * sc;
+                own[3] = o + (double)__half2float(__ushort_as_half((unsigned short)(ownh.y >> 16))) * sc;
             }
             double hv[2] = {0.0, 0.0};
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
                 if (k < nh) {
                     const int rg = hreg[k];
-                    if (uni || L.ntype[g][rg] == 2) hv[k] = f64[hoff[k]];
-                    else hv[k] = L.noff[g][rg] + (double)__half2float(f16[hoff[k]]) * (double)L.nscl[g][rg];
+                    if (uni || L.ntype[g][rg] == 2) hv[k] = __longlong_as_double((long long)hraw[k]);
+                    else
+                        hv[k] = L.noff[g][rg] +
+                                (double)__half2float(__ushort_as_half((unsigned short)hraw[k])) * (double)L.nscl[g][rg];
                 }
             }
             double* row = L.box + (ly + 1) * LWD + C0 + x0;
@@ -379,6 +403,7 @@ __global__ __launch_bounds__(64) void monitor_grad_wave16(StepArgs a, const Inst
 #pragma unroll
             for (int k = 0; k < 2; ++k)
                 if (k < nh) L.box[hidx[k]] = hv[k];
+            if (g + 1 < ngroups_eval) load(g + 1);   // next field in flight during this one
             __syncthreads();
             double m = 0.0;
             {
