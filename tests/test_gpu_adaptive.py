@@ -144,6 +144,34 @@ def test_all_low_map_equals_pure_half2_bitwise(amps, O):
         assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("tile", [16, 32])
+@pytest.mark.parametrize("amp", [0.0, 2.0 ** -130, 2.0 ** -110])
+def test_low_path_subnormal_gradients_stay_finite(amps, O, tile, amp):
+    """Low tiles whose phi deviations are zero or subnormal in single
+    precision (phi = amp * noise beside a half at 0.5, so the tiles next to
+    the step are not in the bulk-hold state): the anisotropy's direction
+    must stay finite for gradients far below 1/FLT_MAX, and the step must
+    match the FP64 step."""
+    n = 64
+    rng = np.random.default_rng(5)
+    phi = np.where(np.arange(n)[None, :] < n // 2, amp * rng.random((n, n)), 0.5)
+    T = np.full((n, n), 0.25)
+    out = {}
+    for mode in (0, 1):
+        c, _ = sf_config(amps, O, n, tile=tile, mode=mode, workload=True)
+        c.params[amps.SF["NOISE"]] = 0.0
+        with amps.Context(c) as ctx:
+            ctx.init()
+            ctx.set_field(0, phi)
+            ctx.set_field(1, T)
+            ctx.step(3)
+            ctx.sync()
+            out[mode] = [ctx.get_field(0), ctx.get_field(1)]
+    for g, r in zip(out[1], out[0]):
+        assert np.all(np.isfinite(g))
+        assert np.max(np.abs(g - r)) < 2e-3
+
+
 def test_mixed_static_map_error_between_extremes(amps, O):
     n, t = 128, 16
     c64, _ = sf_config(amps, O, n, tile=t, mode=0, workload=True)

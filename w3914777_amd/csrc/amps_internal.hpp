@@ -111,6 +111,7 @@ struct StepArgs {
     const int* wl_count;        // device count of worklist entries (nullptr: nwork)
     const int* wl_low;          // low-precision tile work-list (snowflake adaptive)
     const int* wl_low_count;
+    int* phi_same;              // [total_tiles] 1: both buffers hold the same phi of a low tile
     const int* type;            // [group][total_tiles] (nullptr: all FP64)
     long long total_cells;
     int total_tiles;
@@ -121,6 +122,7 @@ struct StepArgs {
     int step_add;               // added to *step_ctr
     int ninst;                  // instances in the context (1: descriptor is loop-invariant)
     int grid_cap;               // sweep workgroups at most (0: no cap)
+    int t_halo;                 // snowflake: halo depth of the T Laplacian (2: reference stencil in some instance)
     Buf in, out;
     double* aux_in;             // model auxiliary per-cell array (grain growth: sum of squares)
     double* aux_out;

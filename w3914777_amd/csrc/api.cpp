@@ -83,6 +83,7 @@ struct amps_ctx {
     double* d_scratch = nullptr;           // max instance cells (get/set)
     int* d_wl = nullptr;                   // [2][total_tiles]: FP64 and low tile work-lists
     int* d_wl_count = nullptr;             // [2]
+    int* d_phi_same = nullptr;             // [total_tiles] snowflake: phi identical in both buffers
 
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -238,6 +239,7 @@ StepArgs step_args(amps_ctx* c, int from) {
         a.wl_count = c->d_wl_count;
         a.wl_low = c->d_wl + c->total_tiles;
         a.wl_low_count = c->d_wl_count + 1;
+        a.phi_same = c->d_phi_same;
     }
     a.type = has_low(c) ? c->d_type : nullptr;
     a.total_cells = c->total_cells;
@@ -248,6 +250,9 @@ StepArgs step_args(amps_ctx* c, int from) {
     a.step_ctr = c->d_step;
     a.step_add = 0;
     a.ninst = c->ninst;
+    a.t_halo = 1;
+    for (const amps_config& k : c->cfg)
+        if (k.stencil == AMPS_STENCIL_REF) a.t_halo = 2;
     a.grid_cap = c->grid_cap;
     a.in = c->buf[from];
     a.out = c->buf[from ^ 1];
@@ -309,7 +314,8 @@ int refresh_aux(amps_ctx* c, int b, bool changed_only = false) {
 // Rebuild the FP64 / low tile work-lists from the current types (snowflake).
 int refresh_worklists(amps_ctx* c) {
     if (c->model != AMPS_SNOWFLAKE || !has_low(c)) return AMPS_OK;
-    return launch_worklist(c->d_type, c->total_tiles, c->d_wl, c->d_wl + c->total_tiles, c->d_wl_count, c->stream);
+    return launch_worklist(c->d_type, c->total_tiles, c->d_wl, c->d_wl + c->total_tiles, c->d_wl_count,
+                           c->d_phi_same, c->stream);
 }
 
 // One monitor pass on buffer `b`: monitor values, classification, conversion.
@@ -558,6 +564,7 @@ int amps_create_batch(const amps_config* cfgs, int n, amps_ctx** out) {
     A((void**)&c->d_scratch, (size_t)c->max_inst_cells * sizeof(double));
     A((void**)&c->d_wl, 2 * (size_t)tiles * sizeof(int));
     A((void**)&c->d_wl_count, 2 * sizeof(int));
+    A((void**)&c->d_phi_same, (size_t)tiles * sizeof(int));
     if (rc != AMPS_OK) { std::string m = g_last_error; amps_destroy(c); return fail(rc, m); }
     e = hipMemcpy(c->d_desc, c->desc.data(), sizeof(InstDesc) * n, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(c->d_tile_inst, c->tile_inst.data(), sizeof(int) * tiles, hipMemcpyHostToDevice);
@@ -710,6 +717,7 @@ int amps_set_field(amps_ctx* c, int inst, int field, const double* src, size_t n
                              c->d_type, one, CONVERT_ENCODE_LOW, c->tile, c->stream));
     }
     RCCHK(refresh_aux(c, c->cur));
+    RCCHK(refresh_worklists(c));  // the other buffer no longer holds the same fields
     HIPCHK(hipStreamSynchronize(c->stream));
     return AMPS_OK;
 }

@@ -97,7 +97,7 @@ __device__ void encode_tile(const Buf& b, long long total_cells, int total_tiles
     __syncthreads();
     mn = -block_max_d(-mn, red);
     double off = 0.5 * (mn + mx), hw = 0.5 * (mx - mn);
-    int ex = 0;
+    int ex = hw == 0.0 ? -100 : 0;  // a uniform tile takes the smallest scale
     if (hw > 0.0 && hw <= 1e300) frexp(hw, &ex);
     if (ex < -100) ex = -100;
     if (ex > 100) ex = 100;
@@ -454,37 +454,61 @@ __global__ void classify_kernel(const InstDesc* inst, const int* tile_inst, int 
     }
 }
 
-// Work-lists of FP64 (type 2) and low (type 1) tiles in tile order, built
-// by one workgroup with wave ballots and a block prefix (group 0 types).
-__global__ __launch_bounds__(1024) void worklist_kernel(const int* type, int total_tiles, int* wl_hi, int* wl_lo,
-                                                        int* counts) {
-    __shared__ int sh[16], sl[16];
-    __shared__ int base[2];
+// Work-lists of FP64 (type 2) and low (type 1) tiles in tile order (group 0
+// types).  Workgroup j owns tiles [1024 j, 1024 j + 1024): it counts the
+// entries of all earlier tiles itself (vector loads, a block sum), so no
+// workgroup waits on another, then writes its own entries with wave ballots
+// (consecutive lanes, consecutive tiles: runs of coalesced stores).  The last
+// workgroup records the totals.  Also clears phi_same (when given): every
+// change of types or fields rebuilds the lists, and after one the two
+// buffers of a tile can no longer be assumed equal.
+constexpr int WL_TILES = 1024;
+
+__global__ __launch_bounds__(1024) void worklist_kernel(const int* __restrict__ type, int total_tiles,
+                                                        int* __restrict__ wl_hi, int* __restrict__ wl_lo,
+                                                        int* __restrict__ counts, int* __restrict__ phi_same) {
+    __shared__ int ws[2][16];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid == 0) base[0] = base[1] = 0;
-    __syncthreads();
-    for (int c0 = 0; c0 < total_tiles; c0 += 1024) {
-        const int t = c0 + tid;
-        const int ty = t < total_tiles ? type[t] : 0;
-        const bool hi = ty == 2, lo = ty == 1;
-        const unsigned long long bh = __ballot(hi), bl = __ballot(lo);
-        const unsigned long long below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-        if (lane == 0) { sh[wave] = __popcll(bh); sl[wave] = __popcll(bl); }
-        __syncthreads();
-        int oh = base[0], ol = base[1];
-        for (int w = 0; w < wave; ++w) { oh += sh[w]; ol += sl[w]; }
-        if (hi) wl_hi[oh + __popcll(bh & below)] = t;
-        if (lo) wl_lo[ol + __popcll(bl & below)] = t;
-        __syncthreads();
-        if (tid == 0) {
-            int a = 0, b = 0;
-            for (int w = 0; w < 16; ++w) { a += sh[w]; b += sl[w]; }
-            base[0] += a;
-            base[1] += b;
-        }
-        __syncthreads();
+    const int t0 = blockIdx.x * WL_TILES;
+    int bh = 0, bl = 0;
+    for (int t = tid * 4; t < t0; t += WL_TILES * 4) {
+        const int4 v = *reinterpret_cast<const int4*>(type + t);
+        bh += (v.x == 2) + (v.y == 2) + (v.z == 2) + (v.w == 2);
+        bl += (v.x == 1) + (v.y == 1) + (v.z == 1) + (v.w == 1);
     }
-    if (tid == 0) { counts[0] = base[0]; counts[1] = base[1]; }
+    const int t = t0 + tid;
+    const int ty = t < total_tiles ? type[t] : 0;
+    const bool hi = ty == 2, lo = ty == 1;
+    const unsigned long long mh = __ballot(hi), ml = __ballot(lo);
+    bh = __ockl_wfred_add_i32(bh);
+    bl = __ockl_wfred_add_i32(bl);
+    if (lane == 0) {
+        ws[0][wave] = bh;
+        ws[1][wave] = bl;
+    }
+    __syncthreads();
+    int oh = 0, ol = 0;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) { oh += ws[0][w]; ol += ws[1][w]; }
+    __syncthreads();
+    if (lane == 0) {
+        ws[0][wave] = __popcll(mh);
+        ws[1][wave] = __popcll(ml);
+    }
+    __syncthreads();
+    int th = oh, tl = ol;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) {
+        const int x = ws[0][w], y = ws[1][w];
+        if (w < wave) { oh += x; ol += y; }
+        th += x;
+        tl += y;
+    }
+    const unsigned long long below = (1ull << lane) - 1ull;
+    if (hi) wl_hi[oh + __popcll(mh & below)] = t;
+    if (lo) wl_lo[ol + __popcll(ml & below)] = t;
+    if (phi_same && t < total_tiles) phi_same[t] = 0;
+    if (tid == 0 && t0 + WL_TILES >= total_tiles) { counts[0] = th; counts[1] = tl; }
 }
 
 // Decode every low tile of field f into a dense FP64 array (instance order),
@@ -598,8 +622,10 @@ int launch_gather(const InstDesc* inst, const Buf& b, long long total_cells, int
     return hip_ok();
 }
 
-int launch_worklist(const int* type, int total_tiles, int* wl_hi, int* wl_lo, int* counts, hipStream_t s) {
-    hipLaunchKernelGGL(worklist_kernel, dim3(1), dim3(1024), 0, s, type, total_tiles, wl_hi, wl_lo, counts);
+int launch_worklist(const int* type, int total_tiles, int* wl_hi, int* wl_lo, int* counts, int* phi_same,
+                    hipStream_t s) {
+    const int grid = total_tiles > 0 ? (total_tiles + WL_TILES - 1) / WL_TILES : 1;
+    hipLaunchKernelGGL(worklist_kernel, dim3(grid), dim3(1024), 0, s, type, total_tiles, wl_hi, wl_lo, counts, phi_same);
     return hip_ok();
 }
 

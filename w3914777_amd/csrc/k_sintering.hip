@@ -182,7 +182,7 @@ __global__ __launch_bounds__(256) void sn_step_kernel(StepArgs a, const InstDesc
                 __syncthreads();
                 mn = -block_max_d(-mn, L.red);
                 double off = 0.5 * (mn + mx), hw = 0.5 * (mx - mn);
-                int ex = 0;
+                int ex = hw == 0.0 ? -100 : 0;  // a uniform tile takes the smallest scale
                 if (hw > 0.0) frexp(hw, &ex);
                 if (ex < -100) ex = -100;
                 const float invs = ldexpf(1.0f, 14 - ex);

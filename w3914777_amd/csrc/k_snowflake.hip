@@ -363,8 +363,6 @@ struct LowFetch {
     double hp[G::NHT], ht[G::NHT];                  // halo values: FP64, or the fp16 word's bits
     double offp, offT;
     float sclp, sclT;
-    double nbo0, nbo1;
-    float nbs0, nbs1;
 };
 
 template <int TS>
@@ -407,20 +405,65 @@ __device__ inline void low_fetch(LowFetch<TS>& f, const StepArgs& a, const InstD
     f.offT = a.in.off[a.total_tiles + w];
     f.sclp = a.in.scl[w];
     f.sclT = a.in.scl[a.total_tiles + w];
+}
+
+// LDS of the low sweep: the float boxes, the neighbour offset/scale table
+// (two slots: the next tile's is written while the current tile computes),
+// the per-wave bulk-hold flags and the min/max reduction slots.
+template <int TS>
+struct SfLowLds {
+    using G = SfGeo<TS>;
+    float phi[G::LW * G::LW], T[G::LW * G::LW];
+    float P[G::RSZ], Q[G::RSZ], E2[G::RSZ];
+    double nbo[2][2][9];  // [slot][field][neighbour]
+    float nbs[2][2][9];
+    unsigned hflag[WG / 64];
+    float red[4 * WG / 64];
+};
+
+// Offsets and scales of neighbour tile k = threadIdx.x (threads 0..8) of
+// tile w, both fields.
+struct NbCode {
+    double o0, o1;
+    float s0, s1;
+};
+
+template <int TS>
+__device__ inline void nb_code_fetch(NbCode& c, const StepArgs& a, const InstDesc& d, int w) {
     if (threadIdx.x < 9) {
+        const int lt = w - d.tile_base;
+        const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
         Nb nb;
         nb_setup(nb, d, tile_x, tile_y, TS);
         const int g = nb_tile(nb, threadIdx.x);
-        f.nbo0 = a.in.off[g];
-        f.nbo1 = a.in.off[a.total_tiles + g];
-        f.nbs0 = a.in.scl[g];
-        f.nbs1 = a.in.scl[a.total_tiles + g];
+        c.o0 = a.in.off[g];
+        c.o1 = a.in.off[a.total_tiles + g];
+        c.s0 = a.in.scl[g];
+        c.s1 = a.in.scl[a.total_tiles + g];
     }
 }
 
 template <int TS>
-__device__ void sf_tile_low(SfLds<TS, float>& L, const StepArgs& a, const InstDesc& d, int gtile,
-                            const LowFetch<TS>& f, unsigned hmask, long long step) {
+__device__ inline void nb_code_store(SfLowLds<TS>& L, int slot, const NbCode& c) {
+    if (threadIdx.x < 9) {
+        L.nbo[slot][0][threadIdx.x] = c.o0;
+        L.nbo[slot][1][threadIdx.x] = c.o1;
+        L.nbs[slot][0][threadIdx.x] = c.s0;
+        L.nbs[slot][1][threadIdx.x] = c.s1;
+    }
+}
+
+// One low tile.  Barriers: A (box staged, bulk-hold flags), B (phase 1
+// done; interface tiles only) and C (min/max of the new values).  The
+// neighbour table of the next tile (`nxt`) is stored into slot `slot ^ 1`
+// after A.  There is no barrier at the end of a tile: a thread that starts
+// staging the next tile has passed C, so every read of this tile's boxes is
+// done; the next tile's writes of the flags, the reduction slots and P/Q/E2
+// all come after its own barrier A, which every thread reaches only after
+// its last read of this tile's values.
+template <int TS>
+__device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const StepArgs& a, const InstDesc& d,
+                            int gtile, const LowFetch<TS>& f, unsigned hmask, long long step) {
     using G = SfGeo<TS>;
     constexpr int H = G::H, LW = G::LW, R = G::R;
     const int lt = gtile - d.tile_base;
@@ -430,12 +473,6 @@ __device__ void sf_tile_low(SfLds<TS, float>& L, const StepArgs& a, const InstDe
     const int gcell0 = (oy + ly) * d.nx + ox + x0;
     const double offp = f.offp, offT = f.offT;
     const float sclp = f.sclp, sclT = f.sclT;
-    if (threadIdx.x < 9) {
-        L.nbf[0].off[threadIdx.x] = f.nbo0;
-        L.nbf[1].off[threadIdx.x] = f.nbo1;
-        L.nbf[0].scl[threadIdx.x] = f.nbs0;
-        L.nbf[1].scl[threadIdx.x] = f.nbs1;
-    }
     float pc[R], tc[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) {
@@ -444,7 +481,6 @@ __device__ void sf_tile_low(SfLds<TS, float>& L, const StepArgs& a, const InstDe
     }
     SfHalo<TS> hm;
     sf_halo_map<TS>(hm, d, ox, oy);
-    __syncthreads();  // neighbour table visible
     const float c0 = (float)offp, c1 = (float)(1.0 - offp);
     bool near0 = true, near1 = true;
 #pragma unroll
@@ -465,28 +501,38 @@ __device__ void sf_tile_low(SfLds<TS, float>& L, const StepArgs& a, const InstDe
         } else {
             const unsigned short wp = (unsigned short)__double_as_longlong(f.hp[k]);
             const unsigned short wt = (unsigned short)__double_as_longlong(f.ht[k]);
-            vp = (float)(L.nbf[0].off[r] - offp) + __half2float(__ushort_as_half(wp)) * L.nbf[0].scl[r];
-            L.T[hm.box[k]] = (float)(L.nbf[1].off[r] - offT) + __half2float(__ushort_as_half(wt)) * L.nbf[1].scl[r];
+            vp = (float)(L.nbo[slot][0][r] - offp) + __half2float(__ushort_as_half(wp)) * L.nbs[slot][0][r];
+            L.T[hm.box[k]] =
+                (float)(L.nbo[slot][1][r] - offT) + __half2float(__ushort_as_half(wt)) * L.nbs[slot][1][r];
         }
         L.phi[hm.box[k]] = vp;
         near0 = near0 && fabsf(c0 + vp) <= SF_BULK_EPS;
         near1 = near1 && fabsf(c1 - vp) <= SF_BULK_EPS;
     }
-    const bool hold = __syncthreads_and(near0) || __syncthreads_and(near1);
+    {
+        const unsigned long long b0 = __ballot(near0), b1 = __ballot(near1);
+        if ((threadIdx.x & 63) == 0) L.hflag[threadIdx.x >> 6] = (b0 == ~0ull ? 1u : 0u) | (b1 == ~0ull ? 2u : 0u);
+    }
+    __syncthreads();  // A
+    unsigned fl = L.hflag[0];
+#pragma unroll
+    for (int i = 1; i < WG / 64; ++i) fl &= L.hflag[i];
+    const bool hold = fl != 0u;
+    nb_code_store<TS>(L, slot ^ 1, nxt);
     __half* dphi = a.out.f16 + d.cell_base;
     __half* dT = a.out.f16 + a.total_cells + d.cell_base;
     if (hold) {
         // phi held (same fp16 words, offset and scale); T' = T + dt lap T
         float lt[R];
         sf_lap<TS, float>(L.T, ly, x0, d.stencil == AMPS_STENCIL_REF, d.kf[SFK_INV_DXDY], tc, lt);
-        float mnt = INFINITY, mxt = -INFINITY;
+        float mnt = INFINITY, mxt = -INFINITY, dmn = 0.0f, dmx = 0.0f;
 #pragma unroll
         for (int i = 0; i < R; ++i) {
             tc[i] = tc[i] + d.kf[SFK_DT] * lt[i];
             mnt = fminf(mnt, tc[i]);
             mxt = fmaxf(mxt, tc[i]);
         }
-        block_minmax(mnt, mxt, L.red);
+        block_minmax2(mnt, mxt, dmn, dmx, L.red);  // C
         const TileCode ct = tile_code(mnt, mxt);
         const double offTn = offT + (double)ct.mid;
         unsigned short w[R];
@@ -498,6 +544,7 @@ __device__ void sf_tile_low(SfLds<TS, float>& L, const StepArgs& a, const InstDe
             a.out.scl[gtile] = sclp;
             a.out.off[a.total_tiles + gtile] = offTn;
             a.out.scl[a.total_tiles + gtile] = ct.scl;
+            a.phi_same[gtile] = 1;  // phi copied: both buffers equal
         }
         return;
     }
@@ -505,7 +552,7 @@ __device__ void sf_tile_low(SfLds<TS, float>& L, const StepArgs& a, const InstDe
     sf_phase1<TS, float>(L.phi, L.P, L.Q, L.E2, d, ox, oy, d.kf[SFK_INV_2DX], d.kf[SFK_INV_2DY],
                          d.kf[SFK_COS_N_THETA0], d.kf[SFK_SIN_N_THETA0], d.kf[SFK_EPSB], d.kf[SFK_EPSB_DELTA],
                          d.kf[SFK_MEPSB_ANISO_DELTA]);
-    __syncthreads();
+    __syncthreads();  // B
 
     float rp[R], rt[R];
     sf_phase2<TS, float>(L.phi, L.T, L.P, L.Q, L.E2, ly, x0, pc, tc, rp, rt,
@@ -519,9 +566,8 @@ __device__ void sf_tile_low(SfLds<TS, float>& L, const StepArgs& a, const InstDe
         mnp = fminf(mnp, rp[i]); mxp = fmaxf(mxp, rp[i]);
         mnt = fminf(mnt, rt[i]); mxt = fmaxf(mxt, rt[i]);
     }
-    block_minmax(mnp, mxp, L.red);
+    block_minmax2(mnp, mxp, mnt, mxt, L.red);  // C
     const TileCode cp = tile_code(mnp, mxp);
-    block_minmax(mnt, mxt, L.red);
     const TileCode ct = tile_code(mnt, mxt);
     const double offpn = offp + (double)cp.mid, offTn = offT + (double)ct.mid;
     unsigned short w[R];
@@ -534,6 +580,239 @@ __device__ void sf_tile_low(SfLds<TS, float>& L, const StepArgs& a, const InstDe
         a.out.scl[gtile] = cp.scl;
         a.out.off[a.total_tiles + gtile] = offTn;
         a.out.scl[a.total_tiles + gtile] = ct.scl;
+        a.phi_same[gtile] = 0;
+    }
+}
+
+// ------------------------------------------------------------- hold sweep
+// Low tiles in the bulk-hold state by their tile codes: the tile and its 8
+// neighbours are low and each one's phi code bounds every phi value within
+// SF_BULK_EPS of 0 (or each one's within SF_BULK_EPS of 1): |phi - off| <=
+// 2^14 scl since |h| <= 2^14, so |off| + 2^14 scl <= eps implies |phi| <= eps
+// for every cell.  This implies the per-value test of the full low sweep, so
+// a tile held here would be held there too.  Such a tile keeps phi (same
+// words, offset and scale) and advances only T' = T + dt lap T, with the
+// arithmetic of the full sweep's hold path.  phi_same[tile] records that both
+// buffers already hold that phi (a previous hold step copied it), so the
+// phi words are neither read nor written again.  Tiles that fail the test
+// are left to the full low sweep, which evaluates the same test on the same
+// (unchanged) input codes and takes exactly those tiles.
+//
+// One wave per tile, no LDS barrier: lane l owns R = TS*TS/64 cells of row
+// l / TPR; the T halo is the cross of depth H (no corners: the Laplacian
+// needs none), every element from a low neighbour; the tile's and the
+// neighbours' codes are read one tile ahead by lanes 0..8.
+constexpr double SF_H_BOUND = 16384.0;
+
+template <int TS, int H>
+struct HoldGeo {
+    static constexpr int R = TS * TS / 64;     // cells per lane
+    static constexpr int TPR = TS / R;         // lanes per row
+    static constexpr int C0 = 4;               // box column of tile column 0 (16-byte aligned)
+    static constexpr int LW = TS + 8;          // box row stride
+    static constexpr int ROWS = TS + 2 * H;
+    static constexpr int NX = 4 * TS * H;      // halo cross elements
+    static constexpr int NXL = NX / 64;        // per lane
+    static_assert(NX % 64 == 0, "halo cross must fill whole waves");
+};
+
+struct HoldMeta {
+    int type;
+    double op, ot;
+    float sp, st;
+};
+
+template <int TS>
+__device__ inline void hold_meta_fetch(HoldMeta& m, const StepArgs& a, const InstDesc& d, int w) {
+    const int lane = threadIdx.x & 63;
+    if (lane < 9) {
+        const int lt = w - d.tile_base;
+        const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
+        Nb nb;
+        nb_setup(nb, d, tile_x, tile_y, TS);
+        const int g = nb_tile(nb, lane);
+        m.type = a.type[g];
+        m.op = a.in.off[g];
+        m.ot = a.in.off[a.total_tiles + g];
+        m.sp = a.in.scl[g];
+        m.st = a.in.scl[a.total_tiles + g];
+    }
+}
+
+// Bulk-hold bits of one tile's phi code: bit 0 every value within
+// SF_BULK_EPS of 0, bit 1 within SF_BULK_EPS of 1 (FP64 tiles: neither).  A
+// tile is held when the AND over itself and its 8 neighbours is non-zero.
+__device__ inline unsigned hold_bits(int type, double op, float sp) {
+    if (type != 1) return 0u;
+    const double bnd = SF_H_BOUND * (double)sp, eps = (double)SF_BULK_EPS;
+    return (fabs(op) + bnd <= eps ? 1u : 0u) | (fabs(1.0 - op) + bnd <= eps ? 2u : 0u);
+}
+
+__device__ inline bool hold_of(const HoldMeta& m) {
+    const int lane = threadIdx.x & 63;
+    const unsigned h = lane < 9 ? hold_bits(m.type, m.op, m.sp) : 0u;
+    const unsigned long long n0 = __ballot(h & 1u), n1 = __ballot(h & 2u);
+    return (n0 & 0x1FFull) == 0x1FFull || (n1 & 0x1FFull) == 0x1FFull;
+}
+
+__device__ inline double readlane_d(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xFFFFFFFFll), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ inline float readlane_f(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+
+// R fp16 words of this lane's cells (R = 16: two 16-byte loads; R = 4: one 8-byte load).
+template <int R>
+__device__ inline void load_words(const unsigned short* p, unsigned* wd) {
+    if constexpr (R == 16) {
+        const uint4 u = reinterpret_cast<const uint4*>(p)[0], v = reinterpret_cast<const uint4*>(p)[1];
+        wd[0] = u.x; wd[1] = u.y; wd[2] = u.z; wd[3] = u.w; wd[4] = v.x; wd[5] = v.y; wd[6] = v.z; wd[7] = v.w;
+    } else {
+        static_assert(R == 4, "16x16 or 32x32 tiles");
+        const uint2 u = *reinterpret_cast<const uint2*>(p);
+        wd[0] = u.x; wd[1] = u.y;
+    }
+}
+template <int R>
+__device__ inline void store_words(unsigned short* p, const unsigned* wd) {
+    if constexpr (R == 16) {
+        reinterpret_cast<uint4*>(p)[0] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
+        reinterpret_cast<uint4*>(p)[1] = make_uint4(wd[4], wd[5], wd[6], wd[7]);
+    } else {
+        *reinterpret_cast<uint2*>(p) = make_uint2(wd[0], wd[1]);
+    }
+}
+__device__ inline float word_half(const unsigned* wd, int j) {
+    const unsigned v = wd[j >> 1];
+    return __half2float(__ushort_as_half((unsigned short)((j & 1) ? (v >> 16) : (v & 0xFFFFu))));
+}
+
+template <int TS, int H>
+__device__ void sf_hold_tile(float* Tb, const StepArgs& a, const InstDesc& d, int w, const HoldMeta& m) {
+    using G = HoldGeo<TS, H>;
+    constexpr int R = G::R, LW = G::LW, C0 = G::C0;
+    const int lane = threadIdx.x & 63;
+    const int lt = w - d.tile_base;
+    const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
+    const int ox = tile_x * TS, oy = tile_y * TS;
+    const int ly = lane / G::TPR, x0 = (lane % G::TPR) * R;
+    const int gcell0 = (oy + ly) * d.nx + ox + x0;
+    const bool same = a.phi_same[w] != 0;
+    const unsigned short* iT = reinterpret_cast<const unsigned short*>(a.in.f16 + a.total_cells + d.cell_base);
+    const unsigned short* iP = reinterpret_cast<const unsigned short*>(a.in.f16 + d.cell_base);
+    unsigned tw[R / 2], pw[R / 2];
+    load_words<R>(iT + gcell0, tw);
+    if (!same) load_words<R>(iP + gcell0, pw);
+    // halo cross: band 0 rows above, 1 rows below, 2 columns left, 3 columns right
+    const bool interior = ox >= H && oy >= H && ox + TS + H <= d.nx && oy + TS + H <= d.ny;
+    unsigned short hw[G::NXL];
+    int hbox[G::NXL], hband[G::NXL];
+#pragma unroll
+    for (int k = 0; k < G::NXL; ++k) {
+        const int e = lane + 64 * k, band = e / (TS * H), i = e - band * (TS * H);
+        int gy, gx, by, bx;
+        if (band < 2) {
+            const int r = i / TS, c = i - r * TS;
+            gy = band == 0 ? oy - H + r : oy + TS + r;
+            by = band == 0 ? r : H + TS + r;
+            gx = ox + c;
+            bx = C0 + c;
+        } else {
+            const int r = i / H, c = i - r * H;
+            gy = oy + r;
+            by = H + r;
+            gx = band == 2 ? ox - H + c : ox + TS + c;
+            bx = band == 2 ? C0 - H + c : C0 + TS + c;
+        }
+        if (!interior) { gy = bmap(gy, d.ny, d.bc); gx = bmap(gx, d.nx, d.bc); }
+        hw[k] = iT[gy * d.nx + gx];
+        hbox[k] = by * LW + bx;
+        hband[k] = band;
+    }
+    // codes: own tile (neighbour 4) and the edge neighbours 1, 7, 3, 5
+    const double offT = readlane_d(m.ot, 4), offP = readlane_d(m.op, 4);
+    const float sclT = readlane_f(m.st, 4), sclP = readlane_f(m.sp, 4);
+    const float o1 = (float)(readlane_d(m.ot, 1) - offT), o7 = (float)(readlane_d(m.ot, 7) - offT);
+    const float o3 = (float)(readlane_d(m.ot, 3) - offT), o5 = (float)(readlane_d(m.ot, 5) - offT);
+    const float s1 = readlane_f(m.st, 1), s7 = readlane_f(m.st, 7), s3 = readlane_f(m.st, 3), s5 = readlane_f(m.st, 5);
+    float tc[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        tc[j] = word_half(tw, j) * sclT;
+        Tb[(ly + H) * LW + C0 + x0 + j] = tc[j];
+    }
+#pragma unroll
+    for (int k = 0; k < G::NXL; ++k) {
+        const int bd = hband[k];
+        const float o = bd == 0 ? o1 : (bd == 1 ? o7 : (bd == 2 ? o3 : o5));
+        const float sc = bd == 0 ? s1 : (bd == 1 ? s7 : (bd == 2 ? s3 : s5));
+        Tb[hbox[k]] = o + __half2float(__ushort_as_half(hw[k])) * sc;
+    }
+    __syncthreads();  // one wave: orders the LDS writes before the reads
+    // T' = T + dt lap T, same operand order as the full sweep's Laplacian
+    const int b0 = (ly + H) * LW + C0 + x0;
+    const float inv = d.kf[SFK_INV_DXDY], dt = d.kf[SFK_DT];
+    float mn = INFINITY, mx = -INFINITY;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const float* c = Tb + b0 + j;
+        float l;
+        if (H == 2 && d.stencil == AMPS_STENCIL_REF)
+            l = (c[-1] + c[-2] + c[1] + c[2] + c[-LW] + c[-2 * LW] + c[LW] + c[2 * LW] - 8.0f * c[0]) * inv;
+        else
+            l = (c[-1] + c[1] + c[-LW] + c[LW] - 4.0f * c[0]) * inv;
+        tc[j] = tc[j] + dt * l;
+        mn = fminf(mn, tc[j]);
+        mx = fmaxf(mx, tc[j]);
+    }
+    mn = wave_min(mn);
+    mx = wave_max(mx);
+    const TileCode ct = tile_code(mn, mx);
+    unsigned nw[R / 2];
+#pragma unroll
+    for (int j = 0; j < R; j += 2) {
+        const unsigned lo = __half_as_ushort(__float2half_rn((tc[j] - ct.mid) * ct.inv));
+        const unsigned hi = __half_as_ushort(__float2half_rn((tc[j + 1] - ct.mid) * ct.inv));
+        nw[j >> 1] = lo | (hi << 16);
+    }
+    unsigned short* oT = reinterpret_cast<unsigned short*>(a.out.f16 + a.total_cells + d.cell_base);
+    unsigned short* oP = reinterpret_cast<unsigned short*>(a.out.f16 + d.cell_base);
+    store_words<R>(oT + gcell0, nw);
+    if (!same) store_words<R>(oP + gcell0, pw);
+    if (lane == 0) {
+        a.out.off[a.total_tiles + w] = offT + (double)ct.mid;
+        a.out.scl[a.total_tiles + w] = ct.scl;
+        if (!same) {
+            a.out.off[w] = offP;
+            a.out.scl[w] = sclP;
+            a.phi_same[w] = 1;
+        }
+    }
+    __syncthreads();  // the box is rewritten by the next tile
+}
+
+template <int TS, int H, bool MULTI>
+__global__ __launch_bounds__(64) void sf_hold_kernel(StepArgs a, const InstDesc* __restrict__ inst,
+                                                     const int* __restrict__ tinst, const int* __restrict__ wl,
+                                                     const int* __restrict__ wlc) {
+    using G = HoldGeo<TS, H>;
+    __shared__ __align__(16) float Tb[G::ROWS * G::LW];
+    const int n = *wlc, NG = gridDim.x;
+    int b = blockIdx.x;
+    if (b >= n) return;
+    int w = wl[xcd_remap(b, n)];
+    HoldMeta mc;
+    hold_meta_fetch<TS>(mc, a, inst[MULTI ? tinst[w] : 0], w);
+    for (; b < n; b += NG) {
+        const bool more = b + NG < n;
+        const int wn = more ? wl[xcd_remap(b + NG, n)] : w;
+        HoldMeta mn;
+        hold_meta_fetch<TS>(mn, a, inst[MULTI ? tinst[wn] : 0], wn);
+        if (hold_of(mc)) sf_hold_tile<TS, H>(Tb, a, inst[MULTI ? tinst[w] : 0], w, mc);
+        mc = mn;
+        w = wn;
     }
 }
 
@@ -592,41 +871,98 @@ __global__ __launch_bounds__(256, 3) void sf_high_kernel(StepArgs a, const InstD
     }
 }
 
+// Full low sweep: the low tiles that fail the bulk-hold test of the hold
+// sweep.  A workgroup takes chunks of LOW_CH low-list positions, evaluates
+// the test for all of them in one load round (9 threads per tile, one per
+// neighbour, with the hold sweep's hold_bits on the same input codes),
+// compacts the failing tiles in LDS and updates them one by one.  Chunk c
+// holds the positions c, c + nch, c + 2 nch, ...: failing tiles sit in runs
+// of the (tile-ordered) list, and the stride spreads a run over workgroups.
+constexpr int LOW_CH = 28;  // 28 tiles x 9 neighbours <= 256 threads
+
 template <int TS, bool MULTI>
 __global__ __launch_bounds__(256, 5) void sf_low_kernel(StepArgs a, const InstDesc* __restrict__ inst,
                                                      const int* __restrict__ tinst, const int* __restrict__ wl,
                                                      const int* __restrict__ wlc) {
-    __shared__ __align__(16) SfLds<TS, float> L;
+    __shared__ __align__(16) SfLowLds<TS> L;
+    __shared__ unsigned char hb[LOW_CH * 9];
+    __shared__ int todo[LOW_CH];
+    __shared__ int ntodo;
     const long long step = *a.step_ctr + a.step_add;
     const int n = *wlc, G = gridDim.x;
-    int b = blockIdx.x;
-    if (b >= n) return;
-    int w = wl[xcd_remap(b, n)];
-    int tcur = nb_type_lane<TS>(a, inst[MULTI ? tinst[w] : 0], w);
-    for (; b < n; b += G) {
-        const bool more = b + G < n;
-        const int wn = more ? wl[xcd_remap(b + G, n)] : w;
-        const int tnext = more ? nb_type_lane<TS>(a, inst[MULTI ? tinst[wn] : 0], wn) : 2;
-        const InstDesc& d = inst[MULTI ? tinst[w] : 0];
-        const unsigned hmask = hmask_of(tcur);
-        LowFetch<TS> f;
-        low_fetch<TS>(f, a, d, w, hmask);
-        sf_tile_low<TS>(L, a, d, w, f, hmask, step);
+    const int nch = (n + LOW_CH - 1) / LOW_CH;
+    int slot = 0;
+    for (int ch = blockIdx.x; ch < nch; ch += G) {
+        if (threadIdx.x < LOW_CH * 9) {
+            const int p = ch + (threadIdx.x / 9) * nch;
+            unsigned v = 0u;
+            if (p < n) {
+                const int w = wl[p];
+                const InstDesc& d = inst[MULTI ? tinst[w] : 0];
+                const int lt = w - d.tile_base;
+                const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
+                Nb nb;
+                nb_setup(nb, d, tile_x, tile_y, TS);
+                const int g = nb_tile(nb, threadIdx.x % 9);
+                v = hold_bits(a.type[g], a.in.off[g], a.in.scl[g]);
+            }
+            hb[threadIdx.x] = (unsigned char)v;
+        }
         __syncthreads();
-        w = wn;
-        tcur = tnext;
+        if (threadIdx.x < 64) {
+            const int i = threadIdx.x, p = ch + i * nch;
+            bool full = false;
+            if (i < LOW_CH && p < n) {
+                unsigned m = 3u;
+#pragma unroll
+                for (int k = 0; k < 9; ++k) m &= hb[i * 9 + k];
+                full = m == 0u;
+            }
+            const unsigned long long bal = __ballot(full);
+            if (full) todo[__popcll(bal & ((1ull << i) - 1ull))] = wl[p];
+            if (i == 0) ntodo = __popcll(bal);
+        }
+        __syncthreads();
+        const int nt = ntodo;
+        if (nt > 0) {
+            int w = todo[0];
+            int tcur = nb_type_lane<TS>(a, inst[MULTI ? tinst[w] : 0], w);
+            {
+                NbCode c0;
+                nb_code_fetch<TS>(c0, a, inst[MULTI ? tinst[w] : 0], w);
+                nb_code_store<TS>(L, slot, c0);
+                __syncthreads();
+            }
+            for (int k = 0; k < nt; ++k) {
+                const bool more = k + 1 < nt;
+                const int wn = more ? todo[k + 1] : w;
+                const InstDesc& dn = inst[MULTI ? tinst[wn] : 0];
+                const int tnext = more ? nb_type_lane<TS>(a, dn, wn) : 2;
+                NbCode cn;
+                nb_code_fetch<TS>(cn, a, dn, wn);
+                const InstDesc& d = inst[MULTI ? tinst[w] : 0];
+                const unsigned hmask = hmask_of(tcur);
+                LowFetch<TS> f;
+                low_fetch<TS>(f, a, d, w, hmask);
+                sf_tile_low<TS>(L, slot, cn, a, d, w, f, hmask, step);
+                slot ^= 1;
+                w = wn;
+                tcur = tnext;
+            }
+        }
+        __syncthreads();  // hb, todo and ntodo are rewritten by the next chunk
     }
 }
 
 // ------------------------------------------------------------ host launch
 // Persistent grid of kernel K: resident workgroups per CU (occupancy query,
 // cached per kernel) x CUs, at most one workgroup per work item.
-template <auto K>
+template <auto K, int BS = WG>
 static int resident_grid(int nwork, int cap) {
     static int per_cu = 0, cus = 0;
     if (per_cu == 0) {
         int v = 0, dev = 0, c = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, K, WG, 0) != hipSuccess || v <= 0) v = 2;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, K, BS, 0) != hipSuccess || v <= 0) v = 2;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0)
             c = 256;
@@ -644,11 +980,20 @@ static void sf_launch(const StepArgs& a, bool adaptive, hipStream_t s, hipEvent_
     if (adaptive) {
         constexpr auto kh = sf_high_kernel<TS, true, MULTI>;
         constexpr auto kl = sf_low_kernel<TS, MULTI>;
-        hipLaunchKernelGGL(kh, dim3(resident_grid<kh>(a.total_tiles, a.grid_cap)), blk, 0, s, a, a.inst, a.tile_inst, a.worklist,
-                           a.wl_count);
+        hipLaunchKernelGGL(kh, dim3(resident_grid<kh>(a.total_tiles, a.grid_cap)), blk, 0, s, a, a.inst, a.tile_inst,
+                           a.worklist, a.wl_count);
         if (mid) (void)hipEventRecord(mid, s);
-        hipLaunchKernelGGL(kl, dim3(resident_grid<kl>(a.total_tiles, a.grid_cap)), blk, 0, s, a, a.inst, a.tile_inst, a.wl_low,
-                           a.wl_low_count);
+        if (a.t_halo == 2) {
+            constexpr auto kb = sf_hold_kernel<TS, 2, MULTI>;
+            hipLaunchKernelGGL(kb, dim3(resident_grid<kb, 64>(a.total_tiles, 0)), dim3(64), 0, s, a, a.inst, a.tile_inst,
+                               a.wl_low, a.wl_low_count);
+        } else {
+            constexpr auto kb = sf_hold_kernel<TS, 1, MULTI>;
+            hipLaunchKernelGGL(kb, dim3(resident_grid<kb, 64>(a.total_tiles, 0)), dim3(64), 0, s, a, a.inst, a.tile_inst,
+                               a.wl_low, a.wl_low_count);
+        }
+        hipLaunchKernelGGL(kl, dim3(resident_grid<kl>((a.total_tiles + LOW_CH - 1) / LOW_CH, a.grid_cap)), blk, 0, s, a,
+                           a.inst, a.tile_inst, a.wl_low, a.wl_low_count);
     } else {
         constexpr auto kh = sf_high_kernel<TS, false, MULTI>;
         hipLaunchKernelGGL(kh, dim3(resident_grid<kh>(a.nwork, a.grid_cap)), blk, 0, s, a, a.inst, a.tile_inst,

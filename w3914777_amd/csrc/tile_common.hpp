@@ -202,21 +202,14 @@ __device__ inline double rcp_nr(double x) {
 }
 
 // ----------------------------------------------------------- reductions
-__device__ inline float wave_min(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ inline float wave_max(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ inline double wave_max_d(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
+// Wave-wide reductions (DPP, result in every lane) from the device library.
+extern "C" __device__ float __ockl_wfred_min_f32(float);
+extern "C" __device__ float __ockl_wfred_max_f32(float);
+extern "C" __device__ double __ockl_wfred_max_f64(double);
+
+__device__ inline float wave_min(float v) { return __ockl_wfred_min_f32(v); }
+__device__ inline float wave_max(float v) { return __ockl_wfred_max_f32(v); }
+__device__ inline double wave_max_d(double v) { return __ockl_wfred_max_f64(v); }
 
 // Min and max over the workgroup; red must hold 2*WG/64 floats.  Contains
 // barriers: every thread must call it.
@@ -231,6 +224,26 @@ __device__ inline void block_minmax(float& mn, float& mx, float* red) {
     mx = fmaxf(fmaxf(red[4], red[5]), fmaxf(red[6], red[7]));
 }
 
+// Min and max of two values over the workgroup with a single barrier; red
+// must hold 4*WG/64 floats.  No leading barrier: the caller guarantees that
+// every thread has passed a barrier since it last read red.
+__device__ inline void block_minmax2(float& mn0, float& mx0, float& mn1, float& mx1, float* red) {
+    static_assert(WG == 256, "four waves per workgroup");
+    mn0 = wave_min(mn0);
+    mx0 = wave_max(mx0);
+    mn1 = wave_min(mn1);
+    mx1 = wave_max(mx1);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) *reinterpret_cast<float4*>(red + 4 * w) = make_float4(mn0, mx0, mn1, mx1);
+    __syncthreads();
+    const float4 r0 = reinterpret_cast<const float4*>(red)[0], r1 = reinterpret_cast<const float4*>(red)[1];
+    const float4 r2 = reinterpret_cast<const float4*>(red)[2], r3 = reinterpret_cast<const float4*>(red)[3];
+    mn0 = fminf(fminf(r0.x, r1.x), fminf(r2.x, r3.x));
+    mx0 = fmaxf(fmaxf(r0.y, r1.y), fmaxf(r2.y, r3.y));
+    mn1 = fminf(fminf(r0.z, r1.z), fminf(r2.z, r3.z));
+    mx1 = fmaxf(fmaxf(r0.w, r1.w), fmaxf(r2.w, r3.w));
+}
+
 __device__ inline double block_max_d(double v, double* red) {
     v = wave_max_d(v);
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -243,7 +256,9 @@ __device__ inline double block_max_d(double v, double* red) {
 // ------------------------------------------------------------ fp16 codec
 // A low tile stores h = (value - off) / scl in fp16 (round to nearest even),
 // where off is the tile's mid-range and scl = 2^(e-14) for a half-range
-// below 2^e, so |h| < 2^14 (fp16 max 65504) and the scaling is exact.
+// below 2^e, so |h| < 2^14 (fp16 max 65504) and the scaling is exact.  e is
+// clamped to >= -100, and a uniform tile (half-range 0, every h = 0) takes
+// e = -100, so that |off| + 2^14 scl bounds |value| tightly for it too.
 struct TileCode {
     float mid;   // added to the old offset
     float inv;   // 1/scl
@@ -254,7 +269,7 @@ __device__ inline TileCode tile_code(float mn, float mx) {
     TileCode t;
     t.mid = 0.5f * (mn + mx);
     float hw = 0.5f * (mx - mn);
-    int ex = 0;
+    int ex = hw == 0.0f ? -100 : 0;  // a uniform tile takes the smallest scale
     if (hw > 0.0f && hw <= 3.0e38f) frexpf(hw, &ex);
     if (ex < -100) ex = -100;
     t.scl = ldexpf(1.0f, ex - 14);
