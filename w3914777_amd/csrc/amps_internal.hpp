@@ -111,7 +111,8 @@ struct StepArgs {
     const int* wl_count;        // device count of worklist entries (nullptr: nwork)
     const int* wl_low;          // low-precision tile work-list (snowflake adaptive)
     const int* wl_low_count;
-    int* phi_same;              // [total_tiles] 1: both buffers hold the same phi of a low tile
+    int* same_flags;            // [total_tiles] snowflake low tiles: bit 0 both buffers hold the same phi,
+                                // bit 1 the same T (words, offset and scale)
     const int* type;            // [group][total_tiles] (nullptr: all FP64)
     long long total_cells;
     int total_tiles;

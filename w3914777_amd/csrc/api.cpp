@@ -83,7 +83,7 @@ struct amps_ctx {
     double* d_scratch = nullptr;           // max instance cells (get/set)
     int* d_wl = nullptr;                   // [2][total_tiles]: FP64 and low tile work-lists
     int* d_wl_count = nullptr;             // [2]
-    int* d_phi_same = nullptr;             // [total_tiles] snowflake: phi identical in both buffers
+    int* d_same_flags = nullptr;           // [total_tiles] snowflake: fields identical in both buffers
 
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -239,7 +239,7 @@ StepArgs step_args(amps_ctx* c, int from) {
         a.wl_count = c->d_wl_count;
         a.wl_low = c->d_wl + c->total_tiles;
         a.wl_low_count = c->d_wl_count + 1;
-        a.phi_same = c->d_phi_same;
+        a.same_flags = c->d_same_flags;
     }
     a.type = has_low(c) ? c->d_type : nullptr;
     a.total_cells = c->total_cells;
@@ -315,7 +315,7 @@ int refresh_aux(amps_ctx* c, int b, bool changed_only = false) {
 int refresh_worklists(amps_ctx* c) {
     if (c->model != AMPS_SNOWFLAKE || !has_low(c)) return AMPS_OK;
     return launch_worklist(c->d_type, c->total_tiles, c->d_wl, c->d_wl + c->total_tiles, c->d_wl_count,
-                           c->d_phi_same, c->stream);
+                           c->d_same_flags, c->stream);
 }
 
 // One monitor pass on buffer `b`: monitor values, classification, conversion.
@@ -564,7 +564,7 @@ int amps_create_batch(const amps_config* cfgs, int n, amps_ctx** out) {
     A((void**)&c->d_scratch, (size_t)c->max_inst_cells * sizeof(double));
     A((void**)&c->d_wl, 2 * (size_t)tiles * sizeof(int));
     A((void**)&c->d_wl_count, 2 * sizeof(int));
-    A((void**)&c->d_phi_same, (size_t)tiles * sizeof(int));
+    A((void**)&c->d_same_flags, (size_t)tiles * sizeof(int));
     if (rc != AMPS_OK) { std::string m = g_last_error; amps_destroy(c); return fail(rc, m); }
     e = hipMemcpy(c->d_desc, c->desc.data(), sizeof(InstDesc) * n, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(c->d_tile_inst, c->tile_inst.data(), sizeof(int) * tiles, hipMemcpyHostToDevice);

@@ -113,7 +113,7 @@ __device__ void encode_tile(const Buf& b, long long total_cells, int total_tiles
     }
     if (threadIdx.x == 0) {
         b.off[(size_t)field * total_tiles + gtile] = off;
-        b.scl[(size_t)field * total_tiles + gtile] = ldexpf(1.0f, ex - 14);
+        b.scl[(size_t)field * total_tiles + gtile] = hw == 0.0 ? 0.0f : ldexpf(1.0f, ex - 14);  // 0: uniform
     }
 }
 
@@ -459,14 +459,14 @@ __global__ void classify_kernel(const InstDesc* inst, const int* tile_inst, int 
 // entries of all earlier tiles itself (vector loads, a block sum), so no
 // workgroup waits on another, then writes its own entries with wave ballots
 // (consecutive lanes, consecutive tiles: runs of coalesced stores).  The last
-// workgroup records the totals.  Also clears phi_same (when given): every
+// workgroup records the totals.  Also clears same_flags (when given): every
 // change of types or fields rebuilds the lists, and after one the two
 // buffers of a tile can no longer be assumed equal.
 constexpr int WL_TILES = 1024;
 
 __global__ __launch_bounds__(1024) void worklist_kernel(const int* __restrict__ type, int total_tiles,
                                                         int* __restrict__ wl_hi, int* __restrict__ wl_lo,
-                                                        int* __restrict__ counts, int* __restrict__ phi_same) {
+                                                        int* __restrict__ counts, int* __restrict__ same_flags) {
     __shared__ int ws[2][16];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int t0 = blockIdx.x * WL_TILES;
@@ -507,7 +507,7 @@ __global__ __launch_bounds__(1024) void worklist_kernel(const int* __restrict__ 
     const unsigned long long below = (1ull << lane) - 1ull;
     if (hi) wl_hi[oh + __popcll(mh & below)] = t;
     if (lo) wl_lo[ol + __popcll(ml & below)] = t;
-    if (phi_same && t < total_tiles) phi_same[t] = 0;
+    if (same_flags && t < total_tiles) same_flags[t] = 0;
     if (tid == 0 && t0 + WL_TILES >= total_tiles) { counts[0] = th; counts[1] = tl; }
 }
 
@@ -622,10 +622,10 @@ int launch_gather(const InstDesc* inst, const Buf& b, long long total_cells, int
     return hip_ok();
 }
 
-int launch_worklist(const int* type, int total_tiles, int* wl_hi, int* wl_lo, int* counts, int* phi_same,
+int launch_worklist(const int* type, int total_tiles, int* wl_hi, int* wl_lo, int* counts, int* same_flags,
                     hipStream_t s) {
     const int grid = total_tiles > 0 ? (total_tiles + WL_TILES - 1) / WL_TILES : 1;
-    hipLaunchKernelGGL(worklist_kernel, dim3(grid), dim3(1024), 0, s, type, total_tiles, wl_hi, wl_lo, counts, phi_same);
+    hipLaunchKernelGGL(worklist_kernel, dim3(grid), dim3(1024), 0, s, type, total_tiles, wl_hi, wl_lo, counts, same_flags);
     return hip_ok();
 }
 

@@ -191,7 +191,7 @@ __global__ __launch_bounds__(256) void sn_step_kernel(StepArgs a, const InstDesc
                 for (int i = 0; i < PER; ++i) o[gcell[i]] = __float2half_rn((float)(v[i] - off) * invs);
                 if (threadIdx.x == 0) {
                     a.out.off[(size_t)(1 + k) * a.total_tiles + w] = off;
-                    a.out.scl[(size_t)(1 + k) * a.total_tiles + w] = ldexpf(1.0f, ex - 14);
+                    a.out.scl[(size_t)(1 + k) * a.total_tiles + w] = hw == 0.0 ? 0.0f : ldexpf(1.0f, ex - 14);
                 }
                 __syncthreads();
             }

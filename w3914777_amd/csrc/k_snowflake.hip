@@ -327,6 +327,7 @@ __device__ void sf_tile_f64(SfLds<TS, double>& L, const StepArgs& a, const InstD
 // |phi(1-phi)| <= SF_NOISE_SKIP_LOW (its per-step contribution is then below
 // 2e-9).
 constexpr float SF_BULK_EPS = 1.0e-8f;
+constexpr int SAME_PHI = 1, SAME_T = 2;  // same_flags bits: both buffers hold the same phi / T
 constexpr float SF_NOISE_SKIP_LOW = 1.0e-6f;
 
 // Unpack R (= 4) fp16 words of an 8-byte load.
@@ -430,17 +431,17 @@ struct NbCode {
 
 template <int TS>
 __device__ inline void nb_code_fetch(NbCode& c, const StepArgs& a, const InstDesc& d, int w) {
-    if (threadIdx.x < 9) {
-        const int lt = w - d.tile_base;
-        const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
-        Nb nb;
-        nb_setup(nb, d, tile_x, tile_y, TS);
-        const int g = nb_tile(nb, threadIdx.x);
-        c.o0 = a.in.off[g];
-        c.o1 = a.in.off[a.total_tiles + g];
-        c.s0 = a.in.scl[g];
-        c.s1 = a.in.scl[a.total_tiles + g];
-    }
+    // every thread loads (threads 9.. repeat neighbour 8, cache hits): no
+    // branch around the loads, so their wait lands where they are used
+    const int lt = w - d.tile_base;
+    const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
+    Nb nb;
+    nb_setup(nb, d, tile_x, tile_y, TS);
+    const int g = nb_tile(nb, threadIdx.x < 9 ? (int)threadIdx.x : 8);
+    c.o0 = a.in.off[g];
+    c.o1 = a.in.off[a.total_tiles + g];
+    c.s0 = a.in.scl[g];
+    c.s1 = a.in.scl[a.total_tiles + g];
 }
 
 template <int TS>
@@ -544,7 +545,7 @@ __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const 
             a.out.scl[gtile] = sclp;
             a.out.off[a.total_tiles + gtile] = offTn;
             a.out.scl[a.total_tiles + gtile] = ct.scl;
-            a.phi_same[gtile] = 1;  // phi copied: both buffers equal
+            a.same_flags[gtile] = SAME_PHI;  // phi copied: both buffers hold it
         }
         return;
     }
@@ -580,7 +581,7 @@ __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const 
         a.out.scl[gtile] = cp.scl;
         a.out.off[a.total_tiles + gtile] = offTn;
         a.out.scl[a.total_tiles + gtile] = ct.scl;
-        a.phi_same[gtile] = 0;
+        a.same_flags[gtile] = 0;
     }
 }
 
@@ -592,10 +593,18 @@ __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const 
 // for every cell.  This implies the per-value test of the full low sweep, so
 // a tile held here would be held there too.  Such a tile keeps phi (same
 // words, offset and scale) and advances only T' = T + dt lap T, with the
-// arithmetic of the full sweep's hold path.  phi_same[tile] records that both
-// buffers already hold that phi (a previous hold step copied it), so the
-// phi words are neither read nor written again.  Tiles that fail the test
-// are left to the full low sweep, which evaluates the same test on the same
+// arithmetic of the full sweep's hold path.  same_flags[tile] bit 0 records
+// that both buffers already hold that phi (a previous hold step copied it),
+// so the phi words are neither read nor written again.  A held tile is also
+// T-flat when the codes of the tile and of its four edge neighbours bound
+// every T of its Laplacian box within SF_T_FLAT of the tile's offset
+// (|off_n - off| + 2^14 scl_n <= SF_T_FLAT): its T is held as well (the
+// update would change it by less than dt * 8 SF_T_FLAT / dx^2, ~SF_T_FLAT on
+// the workload grids).  Far from the crystal T is flat down to float
+// subnormals; holding it keeps those tiles free of work.  A flat tile is
+// copied once into the other buffer (bit 1) and skipped outright after that:
+// no loads beyond its codes, no stores.  Tiles that fail the hold test are
+// left to the full low sweep, which evaluates the same test on the same
 // (unchanged) input codes and takes exactly those tiles.
 //
 // One wave per tile, no LDS barrier: lane l owns R = TS*TS/64 cells of row
@@ -603,13 +612,18 @@ __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const 
 // needs none), every element from a low neighbour; the tile's and the
 // neighbours' codes are read one tile ahead by lanes 0..8.
 constexpr double SF_H_BOUND = 16384.0;
+constexpr double SF_T_FLAT = 1.0e-12;
 
 template <int TS, int H>
 struct HoldGeo {
     static constexpr int R = TS * TS / 64;     // cells per lane
     static constexpr int TPR = TS / R;         // lanes per row
     static constexpr int C0 = 4;               // box column of tile column 0 (16-byte aligned)
-    static constexpr int LW = TS + 8;          // box row stride
+    // box row stride: a multiple of 4 floats with LW/4 odd, so the 16-byte
+    // reads and writes of a quarter wave (4 rows x 2 row halves) fall on
+    // distinct bank groups
+    static constexpr int LW = TS + 12;
+    static_assert((LW / 4) % 2 == 1, "odd number of 16-byte groups per row");
     static constexpr int ROWS = TS + 2 * H;
     static constexpr int NX = 4 * TS * H;      // halo cross elements
     static constexpr int NXL = NX / 64;        // per lane
@@ -624,19 +638,19 @@ struct HoldMeta {
 
 template <int TS>
 __device__ inline void hold_meta_fetch(HoldMeta& m, const StepArgs& a, const InstDesc& d, int w) {
+    // every lane loads (lanes 9.. repeat neighbour 8): no branch around the
+    // loads, so the wait for them lands one tile later, where they are used
     const int lane = threadIdx.x & 63;
-    if (lane < 9) {
-        const int lt = w - d.tile_base;
-        const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
-        Nb nb;
-        nb_setup(nb, d, tile_x, tile_y, TS);
-        const int g = nb_tile(nb, lane);
-        m.type = a.type[g];
-        m.op = a.in.off[g];
-        m.ot = a.in.off[a.total_tiles + g];
-        m.sp = a.in.scl[g];
-        m.st = a.in.scl[a.total_tiles + g];
-    }
+    const int lt = w - d.tile_base;
+    const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
+    Nb nb;
+    nb_setup(nb, d, tile_x, tile_y, TS);
+    const int g = nb_tile(nb, lane < 9 ? lane : 8);
+    m.type = a.type[g];
+    m.op = a.in.off[g];
+    m.ot = a.in.off[a.total_tiles + g];
+    m.sp = a.in.scl[g];
+    m.st = a.in.scl[a.total_tiles + g];
 }
 
 // Bulk-hold bits of one tile's phi code: bit 0 every value within
@@ -654,6 +668,17 @@ __device__ inline bool hold_of(const HoldMeta& m) {
     const unsigned long long n0 = __ballot(h & 1u), n1 = __ballot(h & 2u);
     return (n0 & 0x1FFull) == 0x1FFull || (n1 & 0x1FFull) == 0x1FFull;
 }
+
+// T-flat: every T of the tile (neighbour 4) and of its edge neighbours 1, 3,
+// 5, 7 lies within SF_T_FLAT of the tile's T offset, by the codes.
+__device__ inline bool t_flat_of(const HoldMeta& m) {
+    const int lane = threadIdx.x & 63;
+    const double own = __shfl(m.ot, 4, 64);
+    const bool edge = lane == 1 || lane == 3 || lane == 4 || lane == 5 || lane == 7;
+    const unsigned long long ok = __ballot(edge && fabs(m.ot - own) + SF_H_BOUND * (double)m.st <= SF_T_FLAT);
+    return (ok & 0xBAull) == 0xBAull;
+}
+
 
 __device__ inline double readlane_d(double v, int l) {
     const long long b = __double_as_longlong(v);
@@ -689,8 +714,31 @@ __device__ inline float word_half(const unsigned* wd, int j) {
     return __half2float(__ushort_as_half((unsigned short)((j & 1) ? (v >> 16) : (v & 0xFFFFu))));
 }
 
+// A held, T-flat tile: both fields keep their words, offsets and scales.
+// Copies whichever field the other buffer does not hold yet.
+template <int TS, int R>
+__device__ void sf_flat_copy(const StepArgs& a, const InstDesc& d, int w, int same) {
+    constexpr int TPR = TS / R;
+    const int lane = threadIdx.x & 63;
+    const int lt = w - d.tile_base;
+    const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
+    const int gcell0 = (tile_y * TS + lane / TPR) * d.nx + tile_x * TS + (lane % TPR) * R;
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+        if (same & (f == 0 ? SAME_PHI : SAME_T)) continue;
+        unsigned wd[R / 2];
+        load_words<R>(reinterpret_cast<const unsigned short*>(a.in.f16 + (size_t)f * a.total_cells + d.cell_base) + gcell0, wd);
+        store_words<R>(reinterpret_cast<unsigned short*>(a.out.f16 + (size_t)f * a.total_cells + d.cell_base) + gcell0, wd);
+        if (lane == 0) {
+            a.out.off[(size_t)f * a.total_tiles + w] = a.in.off[(size_t)f * a.total_tiles + w];
+            a.out.scl[(size_t)f * a.total_tiles + w] = a.in.scl[(size_t)f * a.total_tiles + w];
+        }
+    }
+    if (lane == 0) a.same_flags[w] = SAME_PHI | SAME_T;
+}
+
 template <int TS, int H>
-__device__ void sf_hold_tile(float* Tb, const StepArgs& a, const InstDesc& d, int w, const HoldMeta& m) {
+__device__ void sf_hold_tile(float* Tb, const StepArgs& a, const InstDesc& d, int w, const HoldMeta& m, bool same) {
     using G = HoldGeo<TS, H>;
     constexpr int R = G::R, LW = G::LW, C0 = G::C0;
     const int lane = threadIdx.x & 63;
@@ -699,12 +747,10 @@ __device__ void sf_hold_tile(float* Tb, const StepArgs& a, const InstDesc& d, in
     const int ox = tile_x * TS, oy = tile_y * TS;
     const int ly = lane / G::TPR, x0 = (lane % G::TPR) * R;
     const int gcell0 = (oy + ly) * d.nx + ox + x0;
-    const bool same = a.phi_same[w] != 0;
     const unsigned short* iT = reinterpret_cast<const unsigned short*>(a.in.f16 + a.total_cells + d.cell_base);
     const unsigned short* iP = reinterpret_cast<const unsigned short*>(a.in.f16 + d.cell_base);
     unsigned tw[R / 2], pw[R / 2];
     load_words<R>(iT + gcell0, tw);
-    if (!same) load_words<R>(iP + gcell0, pw);
     // halo cross: band 0 rows above, 1 rows below, 2 columns left, 3 columns right
     const bool interior = ox >= H && oy >= H && ox + TS + H <= d.nx && oy + TS + H <= d.ny;
     unsigned short hw[G::NXL];
@@ -731,6 +777,7 @@ __device__ void sf_hold_tile(float* Tb, const StepArgs& a, const InstDesc& d, in
         hbox[k] = by * LW + bx;
         hband[k] = band;
     }
+    if (!same) load_words<R>(iP + gcell0, pw);  // issued after the T loads: they do not wait on it
     // codes: own tile (neighbour 4) and the edge neighbours 1, 7, 3, 5
     const double offT = readlane_d(m.ot, 4), offP = readlane_d(m.op, 4);
     const float sclT = readlane_f(m.st, 4), sclP = readlane_f(m.sp, 4);
@@ -751,21 +798,44 @@ __device__ void sf_hold_tile(float* Tb, const StepArgs& a, const InstDesc& d, in
         Tb[hbox[k]] = o + __half2float(__ushort_as_half(hw[k])) * sc;
     }
     __syncthreads();  // one wave: orders the LDS writes before the reads
-    // T' = T + dt lap T, same operand order as the full sweep's Laplacian
-    const int b0 = (ly + H) * LW + C0 + x0;
+    // T' = T + dt lap T, same operand order as the full sweep's Laplacian;
+    // every box read is an aligned 16-byte read: the own row as columns
+    // x0-4 .. x0+R+3, the rows above and below as columns x0 .. x0+R-1
+    constexpr int LW4 = LW / 4, Q = R / 4;
+    const float4* B4 = reinterpret_cast<const float4*>(Tb);
+    const int b4 = ((ly + H) * LW + C0 + x0) / 4;
+    float row[R + 8];
+#pragma unroll
+    for (int q = 0; q < Q + 2; ++q) {
+        float4 v = B4[b4 - 1 + q];
+        // keep the whole 16-byte read (only part of the first and last
+        // groups is used; split into 4-byte reads they would conflict)
+        asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w));
+        row[4 * q] = v.x; row[4 * q + 1] = v.y; row[4 * q + 2] = v.z; row[4 * q + 3] = v.w;
+    }
+    const bool ref = H == 2 && d.stencil == AMPS_STENCIL_REF;
     const float inv = d.kf[SFK_INV_DXDY], dt = d.kf[SFK_DT];
     float mn = INFINITY, mx = -INFINITY;
 #pragma unroll
-    for (int j = 0; j < R; ++j) {
-        const float* c = Tb + b0 + j;
-        float l;
-        if (H == 2 && d.stencil == AMPS_STENCIL_REF)
-            l = (c[-1] + c[-2] + c[1] + c[2] + c[-LW] + c[-2 * LW] + c[LW] + c[2 * LW] - 8.0f * c[0]) * inv;
-        else
-            l = (c[-1] + c[1] + c[-LW] + c[LW] - 4.0f * c[0]) * inv;
-        tc[j] = tc[j] + dt * l;
-        mn = fminf(mn, tc[j]);
-        mx = fmaxf(mx, tc[j]);
+    for (int q = 0; q < Q; ++q) {
+        const float4 u1 = B4[b4 - LW4 + q], d1 = B4[b4 + LW4 + q];
+        float4 u2 = u1, d2 = d1;
+        if (ref) { u2 = B4[b4 - 2 * LW4 + q]; d2 = B4[b4 + 2 * LW4 + q]; }
+        const float U1[4] = {u1.x, u1.y, u1.z, u1.w}, D1[4] = {d1.x, d1.y, d1.z, d1.w};
+        const float U2[4] = {u2.x, u2.y, u2.z, u2.w}, D2[4] = {d2.x, d2.y, d2.z, d2.w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int j = 4 * q + i;
+            const float* c = row + 4 + j;  // c[0]: this cell
+            float l;
+            if (ref)
+                l = (c[-1] + c[-2] + c[1] + c[2] + U1[i] + U2[i] + D1[i] + D2[i] - 8.0f * c[0]) * inv;
+            else
+                l = (c[-1] + c[1] + U1[i] + D1[i] - 4.0f * c[0]) * inv;
+            tc[j] = tc[j] + dt * l;
+            mn = fminf(mn, tc[j]);
+            mx = fmaxf(mx, tc[j]);
+        }
     }
     mn = wave_min(mn);
     mx = wave_max(mx);
@@ -787,8 +857,8 @@ __device__ void sf_hold_tile(float* Tb, const StepArgs& a, const InstDesc& d, in
         if (!same) {
             a.out.off[w] = offP;
             a.out.scl[w] = sclP;
-            a.phi_same[w] = 1;
         }
+        a.same_flags[w] = SAME_PHI;  // phi is now in both buffers
     }
     __syncthreads();  // the box is rewritten by the next tile
 }
@@ -805,13 +875,21 @@ __global__ __launch_bounds__(64) void sf_hold_kernel(StepArgs a, const InstDesc*
     int w = wl[xcd_remap(b, n)];
     HoldMeta mc;
     hold_meta_fetch<TS>(mc, a, inst[MULTI ? tinst[w] : 0], w);
+    int same = a.same_flags[w];
     for (; b < n; b += NG) {
+        // the next tile's codes and same_flags load while this tile runs
         const bool more = b + NG < n;
         const int wn = more ? wl[xcd_remap(b + NG, n)] : w;
         HoldMeta mn;
         hold_meta_fetch<TS>(mn, a, inst[MULTI ? tinst[wn] : 0], wn);
-        if (hold_of(mc)) sf_hold_tile<TS, H>(Tb, a, inst[MULTI ? tinst[w] : 0], w, mc);
+        const int same_n = a.same_flags[wn];
+        if (hold_of(mc)) {
+            const InstDesc& d = inst[MULTI ? tinst[w] : 0];
+            if (!t_flat_of(mc)) sf_hold_tile<TS, H>(Tb, a, d, w, mc, (same & SAME_PHI) != 0);
+            else if (same != (SAME_PHI | SAME_T)) sf_flat_copy<TS, G::R>(a, d, w, same);
+        }
         mc = mn;
+        same = same_n;
         w = wn;
     }
 }
@@ -828,18 +906,21 @@ __global__ __launch_bounds__(64) void sf_hold_kernel(StepArgs a, const InstDesc*
 // descriptor is loaded once per workgroup.  The next work-list entry is read
 // one tile ahead.
 // Type of neighbour tile (lane % 64) of tile w, for lanes 0..8 of every wave
-// (1 elsewhere); hmask_of() turns it into the wave-uniform 9-bit mask of FP64
+// (2 elsewhere); hmask_of() turns it into the wave-uniform 9-bit mask of FP64
 // neighbours.  Issued one tile ahead, so the halo loads of a tile never wait
 // on a type load.
 template <int TS>
 __device__ inline int nb_type_lane(const StepArgs& a, const InstDesc& d, int w) {
     const int lane = threadIdx.x & 63;
-    if (lane >= 9 || a.type == nullptr) return 2;
+    if (a.type == nullptr) return 2;
     const int lt = w - d.tile_base;
     const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
     Nb nb;
     nb_setup(nb, d, tile_x, tile_y, TS);
-    return a.type[nb_tile(nb, lane)];
+    // every lane loads (lanes 9.. repeat neighbour 8): no branch around the
+    // load, so the wait for it can be placed exactly one tile later
+    const int t = a.type[nb_tile(nb, lane < 9 ? lane : 8)];
+    return lane < 9 ? t : 2;
 }
 
 __device__ inline unsigned hmask_of(int t) {

@@ -257,8 +257,9 @@ __device__ inline double block_max_d(double v, double* red) {
 // A low tile stores h = (value - off) / scl in fp16 (round to nearest even),
 // where off is the tile's mid-range and scl = 2^(e-14) for a half-range
 // below 2^e, so |h| < 2^14 (fp16 max 65504) and the scaling is exact.  e is
-// clamped to >= -100, and a uniform tile (half-range 0, every h = 0) takes
-// e = -100, so that |off| + 2^14 scl bounds |value| tightly for it too.
+// clamped to >= -100.  A uniform tile (half-range 0, every h = 0) stores
+// scale 0: its values decode to off exactly, |off| + 2^14 scl bounds them
+// tightly, and scl == 0 tells a reader that the tile is uniform.
 struct TileCode {
     float mid;   // added to the old offset
     float inv;   // 1/scl
@@ -272,7 +273,7 @@ __device__ inline TileCode tile_code(float mn, float mx) {
     int ex = hw == 0.0f ? -100 : 0;  // a uniform tile takes the smallest scale
     if (hw > 0.0f && hw <= 3.0e38f) frexpf(hw, &ex);
     if (ex < -100) ex = -100;
-    t.scl = ldexpf(1.0f, ex - 14);
+    t.scl = hw == 0.0f ? 0.0f : ldexpf(1.0f, ex - 14);  // uniform tile: scale 0 marks it
     t.inv = ldexpf(1.0f, 14 - ex);
     return t;
 }
