@@ -227,11 +227,13 @@ int amps_set_kernel_timing(amps_ctx* ctx, int enable);
 /* Number of timed sweep-kernel launches and their summed duration (ms). */
 int amps_get_kernel_time(amps_ctx* ctx, int64_t* launches, double* total_ms);
 
-/* The same split by precision: AMPS_SWEEP_FP64 is the FP64 sweep and
- * AMPS_SWEEP_LOW the low-precision sweep of the snowflake's adaptive step
- * (two kernels per step); for every other step AMPS_SWEEP_FP64 is the whole
- * sweep and AMPS_SWEEP_LOW is zero.  Replaces the per-kernel cudaEvent pairs
- * around kernel1_mix/mix_kernel2 (snowflake/monitor_independent_withoutmonitor.cu:85-127). */
+/* The same split by kernel.  The snowflake's adaptive step runs two: first
+ * the bulk-hold sweep over the low tiles (AMPS_SWEEP_LOW), then one kernel
+ * that updates the FP64 tiles together with the low tiles outside the
+ * bulk-hold state (AMPS_SWEEP_FP64).  For every other step AMPS_SWEEP_FP64
+ * is the whole sweep and AMPS_SWEEP_LOW is zero.  Replaces the per-kernel
+ * cudaEvent pairs around kernel1_mix/mix_kernel2
+ * (snowflake/monitor_independent_withoutmonitor.cu:85-127). */
 enum { AMPS_SWEEP_ALL = 0, AMPS_SWEEP_FP64 = 1, AMPS_SWEEP_LOW = 2 };
 int amps_get_sweep_time(amps_ctx* ctx, int which, int64_t* launches, double* total_ms);
 

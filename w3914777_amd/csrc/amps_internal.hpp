@@ -113,6 +113,9 @@ struct StepArgs {
     const int* wl_low_count;
     int* same_flags;            // [total_tiles] snowflake low tiles: bit 0 both buffers hold the same phi,
                                 // bit 1 the same T (words, offset and scale)
+    int* full_seg;              // snowflake: [8][full_cap] low tiles outside the bulk-hold state this step
+    int* full_cnt;              // [2][8] entries per segment, by step parity
+    int full_cap;
     const int* type;            // [group][total_tiles] (nullptr: all FP64)
     long long total_cells;
     int total_tiles;

@@ -84,6 +84,9 @@ struct amps_ctx {
     int* d_wl = nullptr;                   // [2][total_tiles]: FP64 and low tile work-lists
     int* d_wl_count = nullptr;             // [2]
     int* d_same_flags = nullptr;           // [total_tiles] snowflake: fields identical in both buffers
+    int* d_full_seg = nullptr;             // [8][full_cap] snowflake: low tiles outside the bulk-hold state
+    int* d_full_cnt = nullptr;             // [2][8]
+    int full_cap = 0;
 
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -240,6 +243,9 @@ StepArgs step_args(amps_ctx* c, int from) {
         a.wl_low = c->d_wl + c->total_tiles;
         a.wl_low_count = c->d_wl_count + 1;
         a.same_flags = c->d_same_flags;
+        a.full_seg = c->d_full_seg;
+        a.full_cnt = c->d_full_cnt;
+        a.full_cap = c->full_cap;
     }
     a.type = has_low(c) ? c->d_type : nullptr;
     a.total_cells = c->total_cells;
@@ -565,6 +571,9 @@ int amps_create_batch(const amps_config* cfgs, int n, amps_ctx** out) {
     A((void**)&c->d_wl, 2 * (size_t)tiles * sizeof(int));
     A((void**)&c->d_wl_count, 2 * sizeof(int));
     A((void**)&c->d_same_flags, (size_t)tiles * sizeof(int));
+    c->full_cap = (tiles + 7) / 8;
+    A((void**)&c->d_full_seg, 8 * (size_t)c->full_cap * sizeof(int));
+    A((void**)&c->d_full_cnt, 16 * sizeof(int));
     if (rc != AMPS_OK) { std::string m = g_last_error; amps_destroy(c); return fail(rc, m); }
     e = hipMemcpy(c->d_desc, c->desc.data(), sizeof(InstDesc) * n, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(c->d_tile_inst, c->tile_inst.data(), sizeof(int) * tiles, hipMemcpyHostToDevice);
@@ -614,6 +623,7 @@ int amps_init(amps_ctx* c) {
     HIPCHK(hipMemcpyAsync(c->d_step, &zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
     unsigned long long z4[4] = {0, 0, 0, 0};
     HIPCHK(hipMemcpyAsync(c->d_stats, z4, sizeof(z4), hipMemcpyHostToDevice, c->stream));
+    if (c->d_full_cnt) HIPCHK(hipMemsetAsync(c->d_full_cnt, 0, 16 * sizeof(int), c->stream));  // step 0 appends to parity 0
     if (c->mode == AMPS_PURE_HALF2)
         RCCHK(launch_convert(c->d_desc, c->d_tile_inst, c->buf[0], c->total_cells, c->total_tiles, nullptr, c->d_type,
                              c->gm, CONVERT_ENCODE_LOW, c->tile, c->stream));
@@ -825,7 +835,10 @@ extern "C" int amps_get_sweep_time(amps_ctx* c, int which, int64_t* launches, do
     if (which < AMPS_SWEEP_ALL || which > AMPS_SWEEP_LOW) return fail(AMPS_E_ARG, "which out of range");
     RCCHK(kev_flush(c));
     *launches = c->klaunches;
-    *total_ms = which == AMPS_SWEEP_ALL ? c->kms : c->kms_part[which - 1];
+    // the snowflake's adaptive step times the low (bulk-hold) sweep first
+    const bool low_first = c->model == AMPS_SNOWFLAKE && has_low(c);
+    const int part = (which == AMPS_SWEEP_FP64) == low_first ? 1 : 0;
+    *total_ms = which == AMPS_SWEEP_ALL ? c->kms : c->kms_part[part];
     return AMPS_OK;
 }
 

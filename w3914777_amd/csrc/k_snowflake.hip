@@ -604,8 +604,8 @@ __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const 
 // subnormals; holding it keeps those tiles free of work.  A flat tile is
 // copied once into the other buffer (bit 1) and skipped outright after that:
 // no loads beyond its codes, no stores.  Tiles that fail the hold test are
-// left to the full low sweep, which evaluates the same test on the same
-// (unchanged) input codes and takes exactly those tiles.
+// appended to the mixed sweep's list, which updates them with the full low
+// step beside the FP64 tiles.
 //
 // One wave per tile, no LDS barrier: lane l owns R = TS*TS/64 cells of row
 // l / TPR; the T halo is the cross of depth H (no corners: the Laplacian
@@ -869,6 +869,11 @@ __global__ __launch_bounds__(64) void sf_hold_kernel(StepArgs a, const InstDesc*
                                                      const int* __restrict__ wlc) {
     using G = HoldGeo<TS, H>;
     __shared__ __align__(16) float Tb[G::ROWS * G::LW];
+    const long long step = *a.step_ctr + a.step_add;
+    // the next step appends to the other parity's segments: clear them (the
+    // kernel that read them, the previous step's mixed sweep, has finished)
+    if (blockIdx.x == 0 && threadIdx.x < 8) a.full_cnt[(int)((step + 1) & 1) * 8 + threadIdx.x] = 0;
+    int* fcnt = a.full_cnt + (int)(step & 1) * 8;
     const int n = *wlc, NG = gridDim.x;
     int b = blockIdx.x;
     if (b >= n) return;
@@ -887,6 +892,11 @@ __global__ __launch_bounds__(64) void sf_hold_kernel(StepArgs a, const InstDesc*
             const InstDesc& d = inst[MULTI ? tinst[w] : 0];
             if (!t_flat_of(mc)) sf_hold_tile<TS, H>(Tb, a, d, w, mc, (same & SAME_PHI) != 0);
             else if (same != (SAME_PHI | SAME_T)) sf_flat_copy<TS, G::R>(a, d, w, same);
+        } else if ((threadIdx.x & 63) == 0) {
+            // for the mixed sweep; segment = list position mod 8 (at most
+            // ceil(n / 8) entries each; 8 counters spread the atomics)
+            const int seg = b & 7;
+            a.full_seg[seg * a.full_cap + atomicAdd(fcnt + seg, 1)] = w;
         }
         mc = mn;
         same = same_n;
@@ -952,86 +962,57 @@ __global__ __launch_bounds__(256, 3) void sf_high_kernel(StepArgs a, const InstD
     }
 }
 
-// Full low sweep: the low tiles that fail the bulk-hold test of the hold
-// sweep.  A workgroup takes chunks of LOW_CH low-list positions, evaluates
-// the test for all of them in one load round (9 threads per tile, one per
-// neighbour, with the hold sweep's hold_bits on the same input codes),
-// compacts the failing tiles in LDS and updates them one by one.  Chunk c
-// holds the positions c, c + nch, c + 2 nch, ...: failing tiles sit in runs
-// of the (tile-ordered) list, and the stride spreads a run over workgroups.
-constexpr int LOW_CH = 28;  // 28 tiles x 9 neighbours <= 256 threads
+// Mixed sweep of the adaptive step: the FP64 tiles (FP64 work-list) and the
+// low tiles that failed the hold sweep's bulk-hold test (its 8 segments of
+// this step's parity) are the items of one persistent launch, so the two
+// latency-bound tile chains run side by side.  Items [0, nh) are FP64 tiles,
+// the rest low tiles; the LDS is a union of the two layouts, with a barrier
+// after every item.
+template <int TS>
+union SfMixLds {
+    SfLds<TS, double> h;
+    SfLowLds<TS> l;
+};
 
 template <int TS, bool MULTI>
-__global__ __launch_bounds__(256, 5) void sf_low_kernel(StepArgs a, const InstDesc* __restrict__ inst,
-                                                     const int* __restrict__ tinst, const int* __restrict__ wl,
-                                                     const int* __restrict__ wlc) {
-    __shared__ __align__(16) SfLowLds<TS> L;
-    __shared__ unsigned char hb[LOW_CH * 9];
-    __shared__ int todo[LOW_CH];
-    __shared__ int ntodo;
+__global__ __launch_bounds__(256, 3) void sf_mixed_kernel(StepArgs a, const InstDesc* __restrict__ inst,
+                                                      const int* __restrict__ tinst, const int* __restrict__ wl,
+                                                      const int* __restrict__ wlc, const int* __restrict__ fseg,
+                                                      const int* __restrict__ fcnt2) {
+    __shared__ __align__(16) SfMixLds<TS> U;
     const long long step = *a.step_ctr + a.step_add;
-    const int n = *wlc, G = gridDim.x;
-    const int nch = (n + LOW_CH - 1) / LOW_CH;
-    int slot = 0;
-    for (int ch = blockIdx.x; ch < nch; ch += G) {
-        if (threadIdx.x < LOW_CH * 9) {
-            const int p = ch + (threadIdx.x / 9) * nch;
-            unsigned v = 0u;
-            if (p < n) {
-                const int w = wl[p];
-                const InstDesc& d = inst[MULTI ? tinst[w] : 0];
-                const int lt = w - d.tile_base;
-                const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
-                Nb nb;
-                nb_setup(nb, d, tile_x, tile_y, TS);
-                const int g = nb_tile(nb, threadIdx.x % 9);
-                v = hold_bits(a.type[g], a.in.off[g], a.in.scl[g]);
-            }
-            hb[threadIdx.x] = (unsigned char)v;
-        }
-        __syncthreads();
-        if (threadIdx.x < 64) {
-            const int i = threadIdx.x, p = ch + i * nch;
-            bool full = false;
-            if (i < LOW_CH && p < n) {
-                unsigned m = 3u;
+    const int* fcnt = fcnt2 + (int)(step & 1) * 8;
+    int cnt[8], nf = 0;
 #pragma unroll
-                for (int k = 0; k < 9; ++k) m &= hb[i * 9 + k];
-                full = m == 0u;
-            }
-            const unsigned long long bal = __ballot(full);
-            if (full) todo[__popcll(bal & ((1ull << i) - 1ull))] = wl[p];
-            if (i == 0) ntodo = __popcll(bal);
+    for (int k = 0; k < 8; ++k) { cnt[k] = fcnt[k]; nf += cnt[k]; }
+    const int nh = *wlc, n = nh + nf, G = gridDim.x;
+    for (int i = blockIdx.x; i < n; i += G) {
+        if (i < nh) {
+            const int w = wl[xcd_remap(i, nh)];
+            const InstDesc& d = inst[MULTI ? tinst[w] : 0];
+            const int t = nb_type_lane<TS>(a, d, w);
+            const int lt = w - d.tile_base;
+            const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
+            sf_tile_f64<TS, true>(U.h, a, d, tile_x, tile_y, hmask_of(t), step);
+        } else {
+            // entry j of the concatenated segments
+            int j = i - nh, seg = 0;
+#pragma unroll
+            for (int k = 0; k < 7; ++k)
+                if (seg == k && j >= cnt[k]) { j -= cnt[k]; seg = k + 1; }
+            const int w = fseg[seg * a.full_cap + j];
+            const InstDesc& d = inst[MULTI ? tinst[w] : 0];
+            const int t = nb_type_lane<TS>(a, d, w);
+            NbCode c0;
+            nb_code_fetch<TS>(c0, a, d, w);
+            const unsigned hmask = hmask_of(t);
+            LowFetch<TS> f;
+            low_fetch<TS>(f, a, d, w, hmask);
+            nb_code_store<TS>(U.l, 0, c0);
+            __syncthreads();  // neighbour table visible
+            sf_tile_low<TS>(U.l, 0, c0, a, d, w, f, hmask, step);
         }
-        __syncthreads();
-        const int nt = ntodo;
-        if (nt > 0) {
-            int w = todo[0];
-            int tcur = nb_type_lane<TS>(a, inst[MULTI ? tinst[w] : 0], w);
-            {
-                NbCode c0;
-                nb_code_fetch<TS>(c0, a, inst[MULTI ? tinst[w] : 0], w);
-                nb_code_store<TS>(L, slot, c0);
-                __syncthreads();
-            }
-            for (int k = 0; k < nt; ++k) {
-                const bool more = k + 1 < nt;
-                const int wn = more ? todo[k + 1] : w;
-                const InstDesc& dn = inst[MULTI ? tinst[wn] : 0];
-                const int tnext = more ? nb_type_lane<TS>(a, dn, wn) : 2;
-                NbCode cn;
-                nb_code_fetch<TS>(cn, a, dn, wn);
-                const InstDesc& d = inst[MULTI ? tinst[w] : 0];
-                const unsigned hmask = hmask_of(tcur);
-                LowFetch<TS> f;
-                low_fetch<TS>(f, a, d, w, hmask);
-                sf_tile_low<TS>(L, slot, cn, a, d, w, f, hmask, step);
-                slot ^= 1;
-                w = wn;
-                tcur = tnext;
-            }
-        }
-        __syncthreads();  // hb, todo and ntodo are rewritten by the next chunk
+        __syncthreads();  // the LDS union is rewritten by the next item
     }
 }
 
@@ -1059,11 +1040,9 @@ template <int TS, bool MULTI>
 static void sf_launch(const StepArgs& a, bool adaptive, hipStream_t s, hipEvent_t mid) {
     dim3 blk(WG);
     if (adaptive) {
-        constexpr auto kh = sf_high_kernel<TS, true, MULTI>;
-        constexpr auto kl = sf_low_kernel<TS, MULTI>;
-        hipLaunchKernelGGL(kh, dim3(resident_grid<kh>(a.total_tiles, a.grid_cap)), blk, 0, s, a, a.inst, a.tile_inst,
-                           a.worklist, a.wl_count);
-        if (mid) (void)hipEventRecord(mid, s);
+        // bulk-hold sweep first (it lists the low tiles it does not hold),
+        // then FP64 tiles and those low tiles in one launch
+        constexpr auto km = sf_mixed_kernel<TS, MULTI>;
         if (a.t_halo == 2) {
             constexpr auto kb = sf_hold_kernel<TS, 2, MULTI>;
             hipLaunchKernelGGL(kb, dim3(resident_grid<kb, 64>(a.total_tiles, 0)), dim3(64), 0, s, a, a.inst, a.tile_inst,
@@ -1073,8 +1052,9 @@ static void sf_launch(const StepArgs& a, bool adaptive, hipStream_t s, hipEvent_
             hipLaunchKernelGGL(kb, dim3(resident_grid<kb, 64>(a.total_tiles, 0)), dim3(64), 0, s, a, a.inst, a.tile_inst,
                                a.wl_low, a.wl_low_count);
         }
-        hipLaunchKernelGGL(kl, dim3(resident_grid<kl>((a.total_tiles + LOW_CH - 1) / LOW_CH, a.grid_cap)), blk, 0, s, a,
-                           a.inst, a.tile_inst, a.wl_low, a.wl_low_count);
+        if (mid) (void)hipEventRecord(mid, s);
+        hipLaunchKernelGGL(km, dim3(resident_grid<km>(a.total_tiles, a.grid_cap)), blk, 0, s, a, a.inst, a.tile_inst,
+                           a.worklist, a.wl_count, (const int*)a.full_seg, (const int*)a.full_cnt);
     } else {
         constexpr auto kh = sf_high_kernel<TS, false, MULTI>;
         hipLaunchKernelGGL(kh, dim3(resident_grid<kh>(a.nwork, a.grid_cap)), blk, 0, s, a, a.inst, a.tile_inst,
