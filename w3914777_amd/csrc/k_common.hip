@@ -575,6 +575,153 @@ int launch_convert(const InstDesc* inst, const int* tile_inst, const Buf& b, lon
     return hip_ok();
 }
 
+// Gradient monitor for 32x32 tiles and one monitored field, one wave per
+// tile over a persistent grid: lane l owns the 16 cells of row l/2, columns
+// 16(l%2) .. +15, and two of the 128 halo-cross elements (rows above and
+// below, columns left and right: the central differences need no corners).
+// The codes of the tile and its 8 neighbours are read one tile ahead.  A low
+// tile that is uniform (scale 0), with edge neighbours uniform at the same
+// offset, has every difference exactly 0: its value 0 is written without
+// reading the field.  Otherwise values are decoded and the quotient
+// evaluated exactly as in monitor_grad_kernel, so the values are identical.
+// Box rows are 38 doubles: an odd number of 16-byte groups, so the 16-byte
+// accesses of 8 rows x 2 row halves fall on distinct bank groups.
+struct MonMeta {
+    int type;
+    double off;
+    float scl;
+};
+
+__device__ inline void mon_meta_fetch(MonMeta& m, const StepArgs& a, const InstDesc& d, const int* types, int field,
+                                      int w) {
+    const int lane = threadIdx.x & 63;
+    const int lt = w - d.tile_base, tyi = lt / d.tx, txi = lt - tyi * d.tx;
+    Nb nb;
+    nb_setup(nb, d, txi, tyi, 32);
+    const int g = nb_tile(nb, lane < 9 ? lane : 8);  // every lane loads: no branch around the loads
+    m.type = types ? types[g] : 2;
+    m.off = types ? a.in.off[(size_t)field * a.total_tiles + g] : 0.0;
+    m.scl = types ? a.in.scl[(size_t)field * a.total_tiles + g] : 0.0f;
+}
+
+__device__ inline double mon_rl_d(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xFFFFFFFFll), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__global__ __launch_bounds__(64) void monitor_grad_wave32(StepArgs a, const InstDesc* __restrict__ inst,
+                                                          const int* __restrict__ tinst,
+                                                          const int* __restrict__ types, int field0,
+                                                          double* __restrict__ md) {
+    constexpr int TS = 32, LWD = 38, C0 = 2;
+    __shared__ __align__(16) double box[(TS + 2) * LWD];
+    const int lane = threadIdx.x;
+    const int ly = lane >> 1, x0 = (lane & 1) << 4;
+    const int n = a.total_tiles, NG = gridDim.x;
+    int w = blockIdx.x;
+    if (w >= n) return;
+    MonMeta mc;
+    mon_meta_fetch(mc, a, inst[tinst[w]], types, field0, w);
+    for (; w < n; w += NG) {
+        const int wn = w + NG < n ? w + NG : w;
+        MonMeta mn;
+        mon_meta_fetch(mn, a, inst[tinst[wn]], types, field0, wn);
+        const InstDesc& d = inst[tinst[w]];
+        // uniform low tile between uniform equal edge neighbours: value 0
+        const double own_off = mon_rl_d(mc.off, 4);
+        const bool edge = lane == 1 || lane == 3 || lane == 4 || lane == 5 || lane == 7;
+        const unsigned long long flat = __ballot(edge && mc.type == 1 && mc.scl == 0.0f && mc.off == own_off);
+        if ((flat & 0xBAull) == 0xBAull) {
+            if (lane == 0) md[w] = 0.0;
+            mc = mn;
+            continue;
+        }
+        const int lt = w - d.tile_base, tyi = lt / d.tx, txi = lt - tyi * d.tx;
+        const int ox = txi * TS, oy = tyi * TS;
+        const bool interior = ox >= 1 && oy >= 1 && ox + TS + 1 <= d.nx && oy + TS + 1 <= d.ny;
+        const double* f64 = a.in.f64 + (size_t)field0 * a.total_cells + d.cell_base;
+        const unsigned short* f16 =
+            reinterpret_cast<const unsigned short*>(a.in.f16 + (size_t)field0 * a.total_cells + d.cell_base);
+        const int cell0 = (oy + ly) * d.nx + ox + x0;
+        const int town = __builtin_amdgcn_readlane(mc.type, 4);
+        // loads: own strip (FP64: 8 x 16 bytes, fp16: 2 x 16 bytes) and two halo elements
+        double own[16];
+        uint4 oh0 = make_uint4(0u, 0u, 0u, 0u), oh1 = oh0;
+        if (town == 2) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const double2 v = *reinterpret_cast<const double2*>(f64 + cell0 + 2 * q);
+                own[2 * q] = v.x;
+                own[2 * q + 1] = v.y;
+            }
+        } else {
+            oh0 = reinterpret_cast<const uint4*>(f16 + cell0)[0];
+            oh1 = reinterpret_cast<const uint4*>(f16 + cell0)[1];
+        }
+        unsigned long long hraw[2];
+        int hbox[2], hreg[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int e = lane + 64 * k, band = e >> 5, i = e & 31;
+            int by, bx;
+            if (band == 0) { by = -1; bx = i; }
+            else if (band == 1) { by = TS; bx = i; }
+            else if (band == 2) { by = i; bx = -1; }
+            else { by = i; bx = TS; }
+            int gy = oy + by, gx = ox + bx;
+            if (!interior) { gy = bmap(gy, d.ny, d.bc); gx = bmap(gx, d.nx, d.bc); }
+            hreg[k] = band == 0 ? 1 : (band == 1 ? 7 : (band == 2 ? 3 : 5));
+            hbox[k] = (by + 1) * LWD + C0 + bx;
+            const int off = gy * d.nx + gx;
+            const int rt = __shfl(mc.type, hreg[k], 64);  // this element's region type
+            if (rt == 2) hraw[k] = (unsigned long long)__double_as_longlong(f64[off]);
+            else hraw[k] = f16[off];
+        }
+        if (town != 2) {
+            const double o = own_off, sc = (double)__shfl(mc.scl, 4, 64);
+            const unsigned wd[8] = {oh0.x, oh0.y, oh0.z, oh0.w, oh1.x, oh1.y, oh1.z, oh1.w};
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const unsigned v = wd[j >> 1];
+                own[j] = o + (double)__half2float(__ushort_as_half((unsigned short)((j & 1) ? (v >> 16) : (v & 0xFFFFu)))) * sc;
+            }
+        }
+        double* row = box + (ly + 1) * LWD + C0 + x0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) *reinterpret_cast<double2*>(row + 2 * q) = make_double2(own[2 * q], own[2 * q + 1]);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            // shuffles outside the branch: a lane shuffles in 0 from an inactive lane
+            const int rg = hreg[k];
+            const int rt = __shfl(mc.type, rg, 64);
+            const double ro = __shfl(mc.off, rg, 64);
+            const float rs = __shfl(mc.scl, rg, 64);
+            box[hbox[k]] = rt == 2 ? __longlong_as_double((long long)hraw[k])
+                                   : ro + (double)__half2float(__ushort_as_half((unsigned short)hraw[k])) * (double)rs;
+        }
+        __syncthreads();  // one wave: orders the box writes before the reads
+        const double r = d.p[0] / d.p[1];  // dx / dy (slots 0 and 1 of every model)
+        double m = 0.0;
+        {
+#pragma clang fp contract(off)
+            const double* c = box + (ly + 1) * LWD + C0 + x0;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const double hx = 0.5 * (c[i + 1] - c[i - 1]);
+                const double hy = 0.5 * (c[i + LWD] - c[i - LWD]) * r;
+                const double qv = hx * hx + hy * hy;
+                if (qv > m) m = qv;
+            }
+        }
+        m = wave_max_d(m);
+        if (lane == 0) md[w] = m;
+        __syncthreads();  // the box is rewritten by the next tile
+        mc = mn;
+    }
+}
+
 int launch_monitor(const StepArgs& a, int kind, int tile, int ngroups_eval, int field0, int field_stride,
                    int type_groups, double* md, double* last, hipStream_t s) {
     const int n = ngroups_eval * a.total_tiles;
@@ -584,6 +731,21 @@ int launch_monitor(const StepArgs& a, int kind, int tile, int ngroups_eval, int 
             const int g16 = a.total_tiles > 262144 ? 262144 : a.total_tiles;
             hipLaunchKernelGGL(monitor_grad_wave16, dim3(g16), dim3(64), 0, s, a, a.inst, a.tile_inst, a.type,
                                ngroups_eval, field0, field_stride, type_groups, md);
+        }
+        else if (tile == 32 && ngroups_eval == 1) {
+            static int per_cu = 0, cus = 0;
+            if (per_cu == 0) {
+                int v = 0, dev = 0, c = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, monitor_grad_wave32, 64, 0) != hipSuccess || v <= 0) v = 8;
+                if (hipGetDevice(&dev) != hipSuccess ||
+                    hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0)
+                    c = 256;
+                per_cu = v;
+                cus = c;
+            }
+            const int g32 = per_cu * cus < a.total_tiles ? per_cu * cus : a.total_tiles;
+            hipLaunchKernelGGL(monitor_grad_wave32, dim3(g32 > 0 ? g32 : 1), dim3(64), 0, s, a, a.inst, a.tile_inst,
+                               a.type, field0, md);
         }
         else if (tile == 32) hipLaunchKernelGGL((monitor_grad_kernel<32>), dim3(grid), dim3(256), 0, s, a, ngroups_eval, field0, field_stride, type_groups, md);
         else if (tile == 64) hipLaunchKernelGGL((monitor_grad_kernel<64>), dim3(grid), dim3(256), 0, s, a, ngroups_eval, field0, field_stride, type_groups, md);
