@@ -318,10 +318,13 @@ int refresh_aux(amps_ctx* c, int b, bool changed_only = false) {
 }
 
 // Rebuild the FP64 / low tile work-lists from the current types (snowflake).
-int refresh_worklists(amps_ctx* c) {
+// after_monitor: only the tiles the monitor pass converted (type_old !=
+// type) lose their same_flags; otherwise (fields or types set from the host)
+// every tile does.
+int refresh_worklists(amps_ctx* c, bool after_monitor = false) {
     if (c->model != AMPS_SNOWFLAKE || !has_low(c)) return AMPS_OK;
-    return launch_worklist(c->d_type, c->total_tiles, c->d_wl, c->d_wl + c->total_tiles, c->d_wl_count,
-                           c->d_same_flags, c->stream);
+    return launch_worklist(c->d_type, after_monitor ? c->d_type_old : nullptr, c->total_tiles, c->d_wl,
+                           c->d_wl + c->total_tiles, c->d_wl_count, c->d_same_flags, c->stream);
 }
 
 // One monitor pass on buffer `b`: monitor values, classification, conversion.
@@ -341,7 +344,7 @@ int enqueue_monitor(amps_ctx* c, int b) {
     RCCHK(launch_convert(c->d_desc, c->d_tile_inst, c->buf[b], c->total_cells, c->total_tiles, c->d_type_old,
                          c->d_type, c->gm, CONVERT_CHANGED, c->tile, c->stream));
     if (c->model == AMPS_GRAINGROWTH) RCCHK(refresh_aux(c, b, true));
-    RCCHK(refresh_worklists(c));
+    RCCHK(refresh_worklists(c, true));
     return AMPS_OK;
 }
 

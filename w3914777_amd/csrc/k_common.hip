@@ -459,14 +459,16 @@ __global__ void classify_kernel(const InstDesc* inst, const int* tile_inst, int 
 // entries of all earlier tiles itself (vector loads, a block sum), so no
 // workgroup waits on another, then writes its own entries with wave ballots
 // (consecutive lanes, consecutive tiles: runs of coalesced stores).  The last
-// workgroup records the totals.  Also clears same_flags (when given): every
-// change of types or fields rebuilds the lists, and after one the two
-// buffers of a tile can no longer be assumed equal.
+// workgroup records the totals.  Also clears same_flags (when given) of
+// every tile whose type changed (type_old given: the tiles a monitor pass
+// converted) or of every tile (type_old null: fields or types were set from
+// the host): the two buffers of such a tile can no longer be assumed equal.
 constexpr int WL_TILES = 1024;
 
-__global__ __launch_bounds__(1024) void worklist_kernel(const int* __restrict__ type, int total_tiles,
-                                                        int* __restrict__ wl_hi, int* __restrict__ wl_lo,
-                                                        int* __restrict__ counts, int* __restrict__ same_flags) {
+__global__ __launch_bounds__(1024) void worklist_kernel(const int* __restrict__ type, const int* __restrict__ type_old,
+                                                        int total_tiles, int* __restrict__ wl_hi,
+                                                        int* __restrict__ wl_lo, int* __restrict__ counts,
+                                                        int* __restrict__ same_flags) {
     __shared__ int ws[2][16];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int t0 = blockIdx.x * WL_TILES;
@@ -507,7 +509,7 @@ __global__ __launch_bounds__(1024) void worklist_kernel(const int* __restrict__ 
     const unsigned long long below = (1ull << lane) - 1ull;
     if (hi) wl_hi[oh + __popcll(mh & below)] = t;
     if (lo) wl_lo[ol + __popcll(ml & below)] = t;
-    if (same_flags && t < total_tiles) same_flags[t] = 0;
+    if (same_flags && t < total_tiles && (!type_old || type_old[t] != ty)) same_flags[t] = 0;
     if (tid == 0 && t0 + WL_TILES >= total_tiles) { counts[0] = th; counts[1] = tl; }
 }
 
@@ -784,10 +786,11 @@ int launch_gather(const InstDesc* inst, const Buf& b, long long total_cells, int
     return hip_ok();
 }
 
-int launch_worklist(const int* type, int total_tiles, int* wl_hi, int* wl_lo, int* counts, int* same_flags,
-                    hipStream_t s) {
+int launch_worklist(const int* type, const int* type_old, int total_tiles, int* wl_hi, int* wl_lo, int* counts,
+                    int* same_flags, hipStream_t s) {
     const int grid = total_tiles > 0 ? (total_tiles + WL_TILES - 1) / WL_TILES : 1;
-    hipLaunchKernelGGL(worklist_kernel, dim3(grid), dim3(1024), 0, s, type, total_tiles, wl_hi, wl_lo, counts, same_flags);
+    hipLaunchKernelGGL(worklist_kernel, dim3(grid), dim3(1024), 0, s, type, type_old, total_tiles, wl_hi, wl_lo, counts,
+                       same_flags);
     return hip_ok();
 }
 

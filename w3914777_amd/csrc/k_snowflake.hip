@@ -609,8 +609,7 @@ __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const 
 //
 // One wave per tile, no LDS barrier: lane l owns R = TS*TS/64 cells of row
 // l / TPR; the T halo is the cross of depth H (no corners: the Laplacian
-// needs none), every element from a low neighbour; the tile's and the
-// neighbours' codes are read one tile ahead by lanes 0..8.
+// needs none), every element from a low neighbour.
 constexpr double SF_H_BOUND = 16384.0;
 constexpr double SF_T_FLAT = 1.0e-12;
 
@@ -636,23 +635,6 @@ struct HoldMeta {
     float sp, st;
 };
 
-template <int TS>
-__device__ inline void hold_meta_fetch(HoldMeta& m, const StepArgs& a, const InstDesc& d, int w) {
-    // every lane loads (lanes 9.. repeat neighbour 8): no branch around the
-    // loads, so the wait for them lands one tile later, where they are used
-    const int lane = threadIdx.x & 63;
-    const int lt = w - d.tile_base;
-    const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
-    Nb nb;
-    nb_setup(nb, d, tile_x, tile_y, TS);
-    const int g = nb_tile(nb, lane < 9 ? lane : 8);
-    m.type = a.type[g];
-    m.op = a.in.off[g];
-    m.ot = a.in.off[a.total_tiles + g];
-    m.sp = a.in.scl[g];
-    m.st = a.in.scl[a.total_tiles + g];
-}
-
 // Bulk-hold bits of one tile's phi code: bit 0 every value within
 // SF_BULK_EPS of 0, bit 1 within SF_BULK_EPS of 1 (FP64 tiles: neither).  A
 // tile is held when the AND over itself and its 8 neighbours is non-zero.
@@ -661,24 +643,6 @@ __device__ inline unsigned hold_bits(int type, double op, float sp) {
     const double bnd = SF_H_BOUND * (double)sp, eps = (double)SF_BULK_EPS;
     return (fabs(op) + bnd <= eps ? 1u : 0u) | (fabs(1.0 - op) + bnd <= eps ? 2u : 0u);
 }
-
-__device__ inline bool hold_of(const HoldMeta& m) {
-    const int lane = threadIdx.x & 63;
-    const unsigned h = lane < 9 ? hold_bits(m.type, m.op, m.sp) : 0u;
-    const unsigned long long n0 = __ballot(h & 1u), n1 = __ballot(h & 2u);
-    return (n0 & 0x1FFull) == 0x1FFull || (n1 & 0x1FFull) == 0x1FFull;
-}
-
-// T-flat: every T of the tile (neighbour 4) and of its edge neighbours 1, 3,
-// 5, 7 lies within SF_T_FLAT of the tile's T offset, by the codes.
-__device__ inline bool t_flat_of(const HoldMeta& m) {
-    const int lane = threadIdx.x & 63;
-    const double own = __shfl(m.ot, 4, 64);
-    const bool edge = lane == 1 || lane == 3 || lane == 4 || lane == 5 || lane == 7;
-    const unsigned long long ok = __ballot(edge && fabs(m.ot - own) + SF_H_BOUND * (double)m.st <= SF_T_FLAT);
-    return (ok & 0xBAull) == 0xBAull;
-}
-
 
 __device__ inline double readlane_d(double v, int l) {
     const long long b = __double_as_longlong(v);
@@ -738,7 +702,8 @@ __device__ void sf_flat_copy(const StepArgs& a, const InstDesc& d, int w, int sa
 }
 
 template <int TS, int H>
-__device__ void sf_hold_tile(float* Tb, const StepArgs& a, const InstDesc& d, int w, const HoldMeta& m, bool same) {
+__device__ void sf_hold_tile(float* Tb, const StepArgs& a, const InstDesc& d, int w, const HoldMeta& m, int lb,
+                             bool same) {
     using G = HoldGeo<TS, H>;
     constexpr int R = G::R, LW = G::LW, C0 = G::C0;
     const int lane = threadIdx.x & 63;
@@ -779,11 +744,12 @@ __device__ void sf_hold_tile(float* Tb, const StepArgs& a, const InstDesc& d, in
     }
     if (!same) load_words<R>(iP + gcell0, pw);  // issued after the T loads: they do not wait on it
     // codes: own tile (neighbour 4) and the edge neighbours 1, 7, 3, 5
-    const double offT = readlane_d(m.ot, 4), offP = readlane_d(m.op, 4);
-    const float sclT = readlane_f(m.st, 4), sclP = readlane_f(m.sp, 4);
-    const float o1 = (float)(readlane_d(m.ot, 1) - offT), o7 = (float)(readlane_d(m.ot, 7) - offT);
-    const float o3 = (float)(readlane_d(m.ot, 3) - offT), o5 = (float)(readlane_d(m.ot, 5) - offT);
-    const float s1 = readlane_f(m.st, 1), s7 = readlane_f(m.st, 7), s3 = readlane_f(m.st, 3), s5 = readlane_f(m.st, 5);
+    const double offT = readlane_d(m.ot, lb + 4), offP = readlane_d(m.op, lb + 4);
+    const float sclT = readlane_f(m.st, lb + 4), sclP = readlane_f(m.sp, lb + 4);
+    const float o1 = (float)(readlane_d(m.ot, lb + 1) - offT), o7 = (float)(readlane_d(m.ot, lb + 7) - offT);
+    const float o3 = (float)(readlane_d(m.ot, lb + 3) - offT), o5 = (float)(readlane_d(m.ot, lb + 5) - offT);
+    const float s1 = readlane_f(m.st, lb + 1), s7 = readlane_f(m.st, lb + 7), s3 = readlane_f(m.st, lb + 3),
+                s5 = readlane_f(m.st, lb + 5);
     float tc[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) {
@@ -863,6 +829,14 @@ __device__ void sf_hold_tile(float* Tb, const StepArgs& a, const InstDesc& d, in
     __syncthreads();  // the box is rewritten by the next tile
 }
 
+// Each wave takes a group of HOLD_GT list positions g, g + ngr, g + 2 ngr,
+// ... (ngr groups: the tiles of a group lie far apart, so the active tiles,
+// which cluster in the list, spread over the waves) and reads the codes of
+// all of them in one round: lane 9 j + k holds neighbour k of the group's
+// tile j.  Holding, flatness and the same_flags of the group are decided
+// together; only the tiles that need work touch their fields.
+constexpr int HOLD_GT = 7;  // 7 tiles x 9 neighbours <= 64 lanes
+
 template <int TS, int H, bool MULTI>
 __global__ __launch_bounds__(64) void sf_hold_kernel(StepArgs a, const InstDesc* __restrict__ inst,
                                                      const int* __restrict__ tinst, const int* __restrict__ wl,
@@ -874,33 +848,53 @@ __global__ __launch_bounds__(64) void sf_hold_kernel(StepArgs a, const InstDesc*
     // kernel that read them, the previous step's mixed sweep, has finished)
     if (blockIdx.x == 0 && threadIdx.x < 8) a.full_cnt[(int)((step + 1) & 1) * 8 + threadIdx.x] = 0;
     int* fcnt = a.full_cnt + (int)(step & 1) * 8;
-    const int n = *wlc, NG = gridDim.x;
-    int b = blockIdx.x;
-    if (b >= n) return;
-    int w = wl[xcd_remap(b, n)];
-    HoldMeta mc;
-    hold_meta_fetch<TS>(mc, a, inst[MULTI ? tinst[w] : 0], w);
-    int same = a.same_flags[w];
-    for (; b < n; b += NG) {
-        // the next tile's codes and same_flags load while this tile runs
-        const bool more = b + NG < n;
-        const int wn = more ? wl[xcd_remap(b + NG, n)] : w;
-        HoldMeta mn;
-        hold_meta_fetch<TS>(mn, a, inst[MULTI ? tinst[wn] : 0], wn);
-        const int same_n = a.same_flags[wn];
-        if (hold_of(mc)) {
-            const InstDesc& d = inst[MULTI ? tinst[w] : 0];
-            if (!t_flat_of(mc)) sf_hold_tile<TS, H>(Tb, a, d, w, mc, (same & SAME_PHI) != 0);
-            else if (same != (SAME_PHI | SAME_T)) sf_flat_copy<TS, G::R>(a, d, w, same);
-        } else if ((threadIdx.x & 63) == 0) {
-            // for the mixed sweep; segment = list position mod 8 (at most
-            // ceil(n / 8) entries each; 8 counters spread the atomics)
-            const int seg = b & 7;
-            a.full_seg[seg * a.full_cap + atomicAdd(fcnt + seg, 1)] = w;
+    const int n = *wlc;
+    const int ngr = (n + HOLD_GT - 1) / HOLD_GT;
+    const int lane = threadIdx.x & 63, tj = lane / 9, tk = lane - 9 * tj;
+    for (int g = blockIdx.x; g < ngr; g += gridDim.x) {
+        // tile of this lane's group slot (lane 63 repeats slot 6's tile)
+        const int jj = tj < HOLD_GT ? tj : HOLD_GT - 1;
+        const int pj = g + jj * ngr;
+        const int pc = pj < n ? pj : g;  // an absent slot repeats slot 0 (loaded, never used)
+        const int w = wl[pc];
+        const InstDesc& dj = inst[MULTI ? tinst[w] : 0];
+        HoldMeta m;
+        {
+            const int lt = w - dj.tile_base;
+            const int tile_y = lt / dj.tx, tile_x = lt - tile_y * dj.tx;
+            Nb nb;
+            nb_setup(nb, dj, tile_x, tile_y, TS);
+            const int gt = nb_tile(nb, tk < 9 ? tk : 8);
+            m.type = a.type[gt];
+            m.op = a.in.off[gt];
+            m.ot = a.in.off[a.total_tiles + gt];
+            m.sp = a.in.scl[gt];
+            m.st = a.in.scl[a.total_tiles + gt];
         }
-        mc = mn;
-        same = same_n;
-        w = wn;
+        const int sf = a.same_flags[w];
+        const unsigned hbits = tj < HOLD_GT ? hold_bits(m.type, m.op, m.sp) : 0u;
+        const unsigned long long n0 = __ballot(hbits & 1u), n1 = __ballot(hbits & 2u);
+        // T-flat: |off_T - own off_T| + 2^14 scl_T <= SF_T_FLAT on slots 1, 3, 4, 5, 7 of the tile
+        const double own_ot = __shfl(m.ot, 9 * jj + 4, 64);
+        const bool fe = tj < HOLD_GT && (tk == 1 || tk == 3 || tk == 4 || tk == 5 || tk == 7);
+        const unsigned long long fl = __ballot(fe && fabs(m.ot - own_ot) + SF_H_BOUND * (double)m.st <= SF_T_FLAT);
+        for (int j = 0; j < HOLD_GT; ++j) {
+            const int p = g + j * ngr;
+            if (p >= n) break;
+            const unsigned long long msk = 0x1FFull << (9 * j), fmsk = 0xBAull << (9 * j);
+            const int wt = __builtin_amdgcn_readlane(w, 9 * j);
+            const InstDesc& d = inst[MULTI ? tinst[wt] : 0];
+            if ((n0 & msk) == msk || (n1 & msk) == msk) {
+                const int same = __builtin_amdgcn_readlane(sf, 9 * j);
+                if ((fl & fmsk) != fmsk) sf_hold_tile<TS, H>(Tb, a, d, wt, m, 9 * j, (same & SAME_PHI) != 0);
+                else if (same != (SAME_PHI | SAME_T)) sf_flat_copy<TS, G::R>(a, d, wt, same);
+            } else if (lane == 0) {
+                // for the mixed sweep; segment = list position mod 8 (at most
+                // ceil(n / 8) entries each; 8 counters spread the atomics)
+                const int seg = p & 7;
+                a.full_seg[seg * a.full_cap + atomicAdd(fcnt + seg, 1)] = wt;
+            }
+        }
     }
 }
 

@@ -32,7 +32,8 @@ int launch_advance_step(long long* step_ctr, int n, hipStream_t s);
 
 // mid (may be null) is recorded between the FP64 and the low-precision sweep.
 int launch_sf_step(const StepArgs& a, int tile, bool adaptive, hipStream_t s, hipEvent_t mid = nullptr);
-int launch_worklist(const int* type, int total_tiles, int* wl_hi, int* wl_lo, int* counts, int* same_flags, hipStream_t s);
+int launch_worklist(const int* type, const int* type_old, int total_tiles, int* wl_hi, int* wl_lo, int* counts,
+                    int* same_flags, hipStream_t s);
 int launch_gg_step(const StepArgs& a, int tile, bool adaptive, int grid, hipStream_t s);
 // type_old (may be null): refresh only tiles where some field's type differs from a.type.
 int launch_gg_aux(const StepArgs& a, int tile, const int* type_old, hipStream_t s);
