@@ -12,7 +12,8 @@ fields already resident in HBM; the same W+K steps are run on the plain FP64
 path for the speed ratio and the L2/Linf error of the adaptive result.
 
 Prints one JSON line (metric, value = adaptive cell-updates/s, roofline of
-the adaptive sweep kernel, cpu_baseline of the oracle on a bounded sample).
+the dominant kernel of the adaptive step, cpu_baseline of the oracle on a
+bounded sample).
 """
 import argparse
 import json
@@ -70,12 +71,17 @@ def run_path(amps, cfg, warmup, steps, kernel_pass=True, with_fields=True):
             ctx.init()
             ctx.step(warmup)
             ctx.sync()
+            i0 = ctx.info()
             ctx.set_kernel_timing(True)
             step_chunked(ctx, steps)
             ctx.sync()
             res["k_launches"], res["k_fp64_ms"] = ctx.sweep_time(amps.SWEEP_FP64)
             _, res["k_low_ms"] = ctx.sweep_time(amps.SWEEP_LOW)
-            res["k_info"] = ctx.info()
+            i1 = ctx.info()
+            res["k_info"] = i1
+            # tiles updated by the snowflake's mixed sweep over the timed steps
+            res["k_fp64_tiles"] = i1.fp64_tile_updates - i0.fp64_tile_updates
+            res["k_full_tiles"] = i1.low_full_tile_updates - i0.low_full_tile_updates
     return res
 
 
@@ -100,11 +106,10 @@ def copy_probe():
         hip.hipFree(b)
 
 
-def traffic_from_profiles():
-    """HBM bytes per launch of the low-precision sweep from the committed
-    rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy
-    kernel), or None."""
-    path = os.path.join(ROOT, "profiles", "sf_low_kernel_4096_traffic.json")
+def traffic_from_profiles(name):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), or None."""
+    path = os.path.join(ROOT, "profiles", name + "_4096_traffic.json")
     try:
         with open(path) as f:
             return json.load(f)
@@ -261,29 +266,31 @@ def main():
                                f"{cfg.threshold}, 5-point, zero flux, noise 0.01, {args.warmup}+{args.steps} steps",
                    "grid": [cfg.nx, cfg.ny], "tile": cfg.tile, "threshold": cfg.threshold},
     }
-    # roofline of the dominant sweep kernel of the adaptive step (the
-    # low-precision sweep): algorithmic bytes = 8 B per low cell-update, the
-    # low fraction of each launch averaged over the kernel-timed run
-    ki = amp["k_info"]
-    hi_m = ki.hi_frac_mean
+    # roofline of the dominant kernel of the adaptive step, the mixed sweep
+    # (FP64 tiles plus the low tiles outside the bulk-hold state): its
+    # algorithmic bytes per launch are 32 B per FP64 cell and 8 B per such low
+    # cell, from the device's tile counts over the kernel-timed steps.  The
+    # bulk-hold sweep moves state-dependent bytes (T of the tiles that are not
+    # T-flat, nothing for the rest) and is reported by its time.
     nl = max(amp["k_launches"], 1)
-    low_s = amp["k_low_ms"] / nl / 1e3
-    hi_s = amp["k_fp64_ms"] / nl / 1e3
-    low_bytes = (1.0 - hi_m) * cells * B_FP16
-    hi_bytes = hi_m * cells * B_FP64
-    achieved = low_bytes / low_s / 1e9
-    tr = traffic_from_profiles()
+    tile_cells = cfg.tile * cfg.tile
+    mix_s = amp["k_fp64_ms"] / nl / 1e3
+    hold_s = amp["k_low_ms"] / nl / 1e3
+    fp64_tiles = amp["k_fp64_tiles"] / nl
+    full_tiles = amp["k_full_tiles"] / nl
+    mix_bytes = fp64_tiles * tile_cells * B_FP64 + full_tiles * tile_cells * B_FP16
+    achieved = mix_bytes / mix_s / 1e9
+    tr = traffic_from_profiles("sf_mixed_kernel")
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": achieved / HBM_PEAK_GBS,
                        "traffic": tr["bytes_per_launch"] if tr else None,
-                       "kernel": "sf_low_kernel<32,false>", "avg_launch_us": low_s * 1e6,
-                       "alg_bytes_per_launch": low_bytes, "low_frac": 1.0 - hi_m}
-    out["fp64_sweep_of_adaptive"] = {"kernel": "sf_high_kernel<32,true,false>", "avg_launch_us": hi_s * 1e6,
-                                     "alg_bytes_per_launch": hi_bytes,
-                                     "achieved_GBs": hi_bytes / max(hi_s, 1e-12) / 1e9}
-    out["adaptive_step_sweeps"] = {"avg_us": (low_s + hi_s) * 1e6,
-                                   "alg_bytes_per_step": low_bytes + hi_bytes,
-                                   "achieved_GBs": (low_bytes + hi_bytes) / (low_s + hi_s) / 1e9}
+                       "kernel": "sf_mixed_kernel<32,false>", "avg_launch_us": mix_s * 1e6,
+                       "alg_bytes_per_launch": mix_bytes, "fp64_tiles_per_launch": fp64_tiles,
+                       "low_full_tiles_per_launch": full_tiles}
+    out["adaptive_step_kernels_us"] = {"sf_mixed_kernel<32,false>": mix_s * 1e6,
+                                       "sf_hold_kernel<32,1,false>": hold_s * 1e6,
+                                       "note": "per-launch averages over the timed steps, HIP events on the "
+                                               "context stream (include the dispatch gap)"}
     out["hi_frac_mean"] = amp["info"].hi_frac_mean
     if f64 is not None:
         f64_value = cells * args.steps / f64["wall"]

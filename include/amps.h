@@ -125,6 +125,9 @@ typedef struct {
     double hi_frac_last;     /* fraction of FP64 tiles after the last monitor pass */
     double hi_frac_mean;     /* mean over monitor passes                            */
     double cells;            /* grid points per step, summed over instances        */
+    int64_t fp64_tile_updates;     /* snowflake adaptive steps since amps_init: FP64 tiles
+                                      updated by the mixed sweep, summed over steps    */
+    int64_t low_full_tile_updates; /* ... and low tiles outside the bulk-hold state    */
 } amps_info;
 
 /* ---- configuration ---- */

@@ -102,6 +102,8 @@ struct Buf {
     __half* f16;   // [nfields][total_cells]
     double* off;   // [nfields][total_tiles]  low tiles: value = off + h * scl
     float* scl;    // [nfields][total_tiles]  power of two
+    unsigned long long* zmask;  // grain growth: [total_tiles] bit f = field f is low and exactly 0
+                                // on the tile (a clear bit only means "not known")
 };
 
 struct StepArgs {
@@ -116,6 +118,7 @@ struct StepArgs {
     int* full_seg;              // snowflake: [8][full_cap] low tiles outside the bulk-hold state this step
     int* full_cnt;              // [2][8] entries per segment, by step parity
     int full_cap;
+    unsigned long long* stats;  // [2] FP64 tiles, [3] low tiles updated by the mixed sweep (summed)
     const int* type;            // [group][total_tiles] (nullptr: all FP64)
     long long total_cells;
     int total_tiles;

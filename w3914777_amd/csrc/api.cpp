@@ -77,13 +77,16 @@ struct amps_ctx {
     int* d_type_old = nullptr;
     double* d_md = nullptr;
     double* d_last = nullptr;
-    unsigned long long* d_stats = nullptr;  // [0] sum of FP64 tiles over passes, [1] last pass
+    unsigned long long* d_stats = nullptr;  // [0] sum of FP64 tiles over passes, [1] last pass,
+                                            // [2], [3] tiles of the snowflake mixed sweep (summed)
     long long* d_step = nullptr;
     double* d_aux[2] = {nullptr, nullptr};
     double* d_scratch = nullptr;           // max instance cells (get/set)
     int* d_wl = nullptr;                   // [2][total_tiles]: FP64 and low tile work-lists
     int* d_wl_count = nullptr;             // [2]
-    int* d_same_flags = nullptr;           // [total_tiles] snowflake: fields identical in both buffers
+    int* d_same_flags = nullptr;           // snowflake [total_tiles], grain growth [q][total_tiles]:
+                                           // tile data identical in both buffers
+    size_t same_flags_n = 0;
     int* d_full_seg = nullptr;             // [8][full_cap] snowflake: low tiles outside the bulk-hold state
     int* d_full_cnt = nullptr;             // [2][8]
     int full_cap = 0;
@@ -246,7 +249,9 @@ StepArgs step_args(amps_ctx* c, int from) {
         a.full_seg = c->d_full_seg;
         a.full_cnt = c->d_full_cnt;
         a.full_cap = c->full_cap;
+        a.stats = c->d_stats;
     }
+    if (c->model == AMPS_GRAINGROWTH && has_low(c)) a.same_flags = c->d_same_flags;
     a.type = has_low(c) ? c->d_type : nullptr;
     a.total_cells = c->total_cells;
     a.total_tiles = c->total_tiles;
@@ -317,6 +322,14 @@ int refresh_aux(amps_ctx* c, int b, bool changed_only = false) {
     return launch_gg_aux(a, c->tile, changed_only && a.type ? c->d_type_old : nullptr, c->stream);
 }
 
+// Grain growth: forget every "field is exactly 0" bit (fields or types were
+// set from the host).
+int clear_zmasks(amps_ctx* c) {
+    for (int b = 0; b < 2; ++b)
+        if (c->buf[b].zmask) HIPCHK(hipMemsetAsync(c->buf[b].zmask, 0, (size_t)c->total_tiles * sizeof(unsigned long long), c->stream));
+    return AMPS_OK;
+}
+
 // Rebuild the FP64 / low tile work-lists from the current types (snowflake).
 // after_monitor: only the tiles the monitor pass converted (type_old !=
 // type) lose their same_flags; otherwise (fields or types set from the host)
@@ -343,7 +356,12 @@ int enqueue_monitor(amps_ctx* c, int b) {
                           c->d_type_old, c->d_type, c->d_stats, c->stream));
     RCCHK(launch_convert(c->d_desc, c->d_tile_inst, c->buf[b], c->total_cells, c->total_tiles, c->d_type_old,
                          c->d_type, c->gm, CONVERT_CHANGED, c->tile, c->stream));
-    if (c->model == AMPS_GRAINGROWTH) RCCHK(refresh_aux(c, b, true));
+    if (c->model == AMPS_GRAINGROWTH) {
+        RCCHK(refresh_aux(c, b, true));
+        if (has_low(c))  // converted field-tiles: the buffers no longer agree
+            RCCHK(launch_clear_changed(c->d_type_old, c->d_type, c->gm, c->nfields, c->total_tiles, c->d_same_flags,
+                                       c->buf[b].zmask, c->stream));
+    }
     RCCHK(refresh_worklists(c, true));
     return AMPS_OK;
 }
@@ -560,6 +578,7 @@ int amps_create_batch(const amps_config* cfgs, int n, amps_ctx** out) {
             A((void**)&c->buf[b].f16, nf * cells * sizeof(__half));
             A((void**)&c->buf[b].off, nf * tiles * sizeof(double));
             A((void**)&c->buf[b].scl, nf * tiles * sizeof(float));
+            if (c->model == AMPS_GRAINGROWTH) A((void**)&c->buf[b].zmask, tiles * sizeof(unsigned long long));
         }
         if (c->model == AMPS_GRAINGROWTH) A((void**)&c->d_aux[b], cells * sizeof(double));
     }
@@ -568,12 +587,13 @@ int amps_create_batch(const amps_config* cfgs, int n, amps_ctx** out) {
     A((void**)&c->d_type_old, gt * sizeof(int));
     A((void**)&c->d_md, gt * sizeof(double));
     if (c->mode == AMPS_ADAPTIVE && c->monitor == AMPS_MON_TEMPORAL) A((void**)&c->d_last, gt * 16 * sizeof(double));
-    A((void**)&c->d_stats, 4 * sizeof(unsigned long long));
+    A((void**)&c->d_stats, 8 * sizeof(unsigned long long));
     A((void**)&c->d_step, sizeof(long long));
     A((void**)&c->d_scratch, (size_t)c->max_inst_cells * sizeof(double));
     A((void**)&c->d_wl, 2 * (size_t)tiles * sizeof(int));
     A((void**)&c->d_wl_count, 2 * sizeof(int));
-    A((void**)&c->d_same_flags, (size_t)tiles * sizeof(int));
+    c->same_flags_n = (size_t)(c->model == AMPS_GRAINGROWTH ? c->q : 1) * tiles;
+    A((void**)&c->d_same_flags, c->same_flags_n * sizeof(int));
     c->full_cap = (tiles + 7) / 8;
     A((void**)&c->d_full_seg, 8 * (size_t)c->full_cap * sizeof(int));
     A((void**)&c->d_full_cnt, 16 * sizeof(int));
@@ -624,9 +644,11 @@ int amps_init(amps_ctx* c) {
     HIPCHK(hipMemcpyAsync(c->d_type_old, types.data(), types.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
     long long zero = 0;
     HIPCHK(hipMemcpyAsync(c->d_step, &zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
-    unsigned long long z4[4] = {0, 0, 0, 0};
-    HIPCHK(hipMemcpyAsync(c->d_stats, z4, sizeof(z4), hipMemcpyHostToDevice, c->stream));
+    static const unsigned long long z8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    HIPCHK(hipMemcpyAsync(c->d_stats, z8, sizeof(z8), hipMemcpyHostToDevice, c->stream));
     if (c->d_full_cnt) HIPCHK(hipMemsetAsync(c->d_full_cnt, 0, 16 * sizeof(int), c->stream));  // step 0 appends to parity 0
+    HIPCHK(hipMemsetAsync(c->d_same_flags, 0, c->same_flags_n * sizeof(int), c->stream));
+    RCCHK(clear_zmasks(c));
     if (c->mode == AMPS_PURE_HALF2)
         RCCHK(launch_convert(c->d_desc, c->d_tile_inst, c->buf[0], c->total_cells, c->total_tiles, nullptr, c->d_type,
                              c->gm, CONVERT_ENCODE_LOW, c->tile, c->stream));
@@ -731,6 +753,8 @@ int amps_set_field(amps_ctx* c, int inst, int field, const double* src, size_t n
     }
     RCCHK(refresh_aux(c, c->cur));
     RCCHK(refresh_worklists(c));  // the other buffer no longer holds the same fields
+    HIPCHK(hipMemsetAsync(c->d_same_flags, 0, c->same_flags_n * sizeof(int), c->stream));
+    RCCHK(clear_zmasks(c));
     HIPCHK(hipStreamSynchronize(c->stream));
     return AMPS_OK;
 }
@@ -766,6 +790,8 @@ int amps_set_tile_types(amps_ctx* c, int inst, const int32_t* src, size_t n) {
                          c->d_type, c->gm, CONVERT_CHANGED, c->tile, c->stream));
     RCCHK(refresh_aux(c, c->cur));
     RCCHK(refresh_worklists(c));
+    HIPCHK(hipMemsetAsync(c->d_same_flags, 0, c->same_flags_n * sizeof(int), c->stream));
+    RCCHK(clear_zmasks(c));
     HIPCHK(hipStreamSynchronize(c->stream));
     return AMPS_OK;
 }
@@ -786,10 +812,12 @@ int amps_get_monitor_values(amps_ctx* c, int inst, double* dst, size_t n) {
 int amps_get_info(amps_ctx* c, amps_info* info) {
     if (!c || !info) return fail(AMPS_E_ARG, "NULL argument");
     HIPCHK(hipStreamSynchronize(c->stream));
-    unsigned long long st[4];
+    unsigned long long st[8];
     HIPCHK(hipMemcpy(st, c->d_stats, sizeof(st), hipMemcpyDeviceToHost));
     std::memset(info, 0, sizeof(*info));
     info->steps_done = c->steps_done;
+    info->fp64_tile_updates = (int64_t)st[2];
+    info->low_full_tile_updates = (int64_t)st[3];
     info->monitor_passes = c->monitor_passes;
     int maxt = 0;
     for (auto& d : c->desc) maxt = std::max(maxt, d.tx * d.ty);

@@ -142,23 +142,39 @@ __global__ __launch_bounds__(256) void convert_kernel(const InstDesc* __restrict
                                                        const int* __restrict__ type_old,
                                                        const int* __restrict__ type, GroupMap gm, int mode) {
     __shared__ double red[4];
+    __shared__ int todo[WG];
+    __shared__ int ntodo;
     const int ngw = gm.ngroups * total_tiles;
-    for (int w = blockIdx.x; w < ngw; w += gridDim.x) {
-        const int g = w / total_tiles, gtile = w - g * total_tiles;
-        const int tn = type[w];
-        const int to = type_old ? type_old[w] : tn;
-        bool enc = false, dec = false;
-        if (mode == CONVERT_CHANGED) { enc = (to == 2 && tn == 1); dec = (to == 1 && tn == 2); }
-        else if (mode == CONVERT_ENCODE_LOW) enc = (tn == 1);
-        else if (mode == CONVERT_DECODE_LOW) dec = (tn == 1);
-        if (!enc && !dec) continue;
-        const InstDesc& d = inst[tile_inst[gtile]];
-        for (int f = 0; f < gm.nfields; ++f) {
-            if (gm.fg[f] != g) continue;
-            if (enc) encode_tile<TS>(b, total_cells, total_tiles, d, gtile, f, red);
-            else decode_tile<TS>(b, total_cells, total_tiles, d, gtile, f, b.f64 + (size_t)f * total_cells + d.cell_base);
-            __syncthreads();
+    // chunks of WG items: every thread decides one item, the items that
+    // convert are compacted and converted one by one by the workgroup
+    for (int c0 = blockIdx.x * WG; c0 < ngw; c0 += gridDim.x * WG) {
+        const int w = c0 + (int)threadIdx.x;
+        int op = 0;  // 1 encode, 2 decode
+        if (w < ngw) {
+            const int tn = type[w];
+            const int to = type_old ? type_old[w] : tn;
+            if (mode == CONVERT_CHANGED) op = (to == 2 && tn == 1) ? 1 : ((to == 1 && tn == 2) ? 2 : 0);
+            else if (mode == CONVERT_ENCODE_LOW) op = tn == 1 ? 1 : 0;
+            else if (mode == CONVERT_DECODE_LOW) op = tn == 1 ? 2 : 0;
         }
+        if (threadIdx.x == 0) ntodo = 0;
+        __syncthreads();
+        if (op) todo[atomicAdd(&ntodo, 1)] = w * 2 + (op - 1);  // LDS atomic: order does not matter
+        __syncthreads();
+        const int nt = ntodo;
+        for (int i = 0; i < nt; ++i) {
+            const int it = todo[i], wi = it >> 1;
+            const bool enc = (it & 1) == 0;
+            const int g = wi / total_tiles, gtile = wi - g * total_tiles;
+            const InstDesc& d = inst[tile_inst[gtile]];
+            for (int f = 0; f < gm.nfields; ++f) {
+                if (gm.fg[f] != g) continue;
+                if (enc) encode_tile<TS>(b, total_cells, total_tiles, d, gtile, f, red);
+                else decode_tile<TS>(b, total_cells, total_tiles, d, gtile, f, b.f64 + (size_t)f * total_cells + d.cell_base);
+                __syncthreads();
+            }
+        }
+        __syncthreads();  // todo and ntodo are rewritten by the next chunk
     }
 }
 
@@ -335,11 +351,30 @@ __global__ __launch_bounds__(64) void monitor_grad_wave16(StepArgs a, const Inst
                 hidx[k] = (by + 1) * LWD + C0 + bx;
             }
         }
+        // fields exactly 0 on the tile and its edge neighbours (grain growth's
+        // zero masks): every central difference is 0, so the value is 0
+        unsigned long long z5 = 0ull;
+        if (!uni && a.in.zmask) {
+            Nb nb;
+            nb_setup(nb, d, txi, tyi, TS);
+            const unsigned long long zm = a.in.zmask[nb_tile(nb, lane < 9 ? lane : 8)];
+            z5 = ~0ull;
+            const int kk[5] = {1, 3, 4, 5, 7};  // edge neighbours and the tile
+#pragma unroll
+            for (int i = 0; i < 5; ++i) {
+                const unsigned lo = __builtin_amdgcn_readlane((unsigned)(zm & 0xFFFFFFFFull), kk[i]);
+                const unsigned hi = __builtin_amdgcn_readlane((unsigned)(zm >> 32), kk[i]);
+                z5 &= ((unsigned long long)hi << 32) | lo;
+            }
+            if (field_stride != 1 || field0 != 0) z5 = 0ull;  // bits are fields; groups must be fields 0..
+            if (lane < ngroups_eval && ((z5 >> lane) & 1ull)) md[(size_t)lane * a.total_tiles + w] = 0.0;
+        }
         if (!uni) {
             Nb nb;
             nb_setup(nb, d, txi, tyi, TS);
             for (int e = lane; e < ngroups_eval * 9; e += 64) {
                 const int g = e / 9, k = e - g * 9;
+                if ((z5 >> g) & 1ull) continue;  // value already written
                 const int field = field0 + g * field_stride, tgroup = type_groups > 1 ? g : 0;
                 const int gt = nb_tile(nb, k);
                 L.ntype[g][k] = types[(size_t)tgroup * a.total_tiles + gt];
@@ -374,8 +409,11 @@ __global__ __launch_bounds__(64) void monitor_grad_wave16(StepArgs a, const Inst
                 }
             }
         };
-        if (ngroups_eval > 0) load(0);
+        const unsigned long long live = ~z5;
+        const int g0 = (int)__builtin_ctzll(live | (1ull << 63));
+        if (g0 < ngroups_eval) load(g0);
         for (int g = 0; g < ngroups_eval; ++g) {
+            if ((z5 >> g) & 1ull) continue;
             double own[4];
             if (uni || L.ntype[g][4] == 2) {
                 own[0] = ownd[0]; own[1] = ownd[1]; own[2] = ownd[2]; own[3] = ownd[3];
@@ -403,7 +441,12 @@ __global__ __launch_bounds__(64) void monitor_grad_wave16(StepArgs a, const Inst
 #pragma unroll
             for (int k = 0; k < 2; ++k)
                 if (k < nh) L.box[hidx[k]] = hv[k];
-            if (g + 1 < ngroups_eval) load(g + 1);   // next field in flight during this one
+            {
+                // next evaluated field in flight during this one
+                const int gn = g + 1 < ngroups_eval ? (int)__builtin_ctzll((live >> (g + 1) << (g + 1)) | (1ull << 63))
+                                                    : ngroups_eval;
+                if (gn < ngroups_eval) load(gn);
+            }
             __syncthreads();
             double m = 0.0;
             {
@@ -428,29 +471,63 @@ __global__ __launch_bounds__(64) void monitor_grad_wave16(StepArgs a, const Inst
 // self and the 4 non-periodic von-Neumann neighbours with fmax (CUDA's max);
 // < thr -> 1, >= thr -> 2, unordered keeps the old type.  One thread per
 // (group, tile); with broadcast every group takes group 0's monitor value.
-__global__ void classify_kernel(const InstDesc* inst, const int* tile_inst, int total_tiles, int ngroups,
-                                int broadcast, const double* md, double thr, int* type_old, int* type,
-                                unsigned long long* stats) {
-    const int w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= ngroups * total_tiles) return;
-    const int g = w / total_tiles, gtile = w - g * total_tiles;
-    const InstDesc& d = inst[tile_inst[gtile]];
-    const int lt = gtile - d.tile_base, uy = lt / d.tx, ux = lt - uy * d.tx;
-    const double* m0 = md + (size_t)(broadcast ? 0 : g) * total_tiles;
-    double m = m0[gtile];
-    if (uy < d.ty - 1) m = fmax(m, m0[gtile + d.tx]);
-    if (uy > 0) m = fmax(m, m0[gtile - d.tx]);
-    if (ux < d.tx - 1) m = fmax(m, m0[gtile + 1]);
-    if (ux > 0) m = fmax(m, m0[gtile - 1]);
-    int t = type[w];
-    type_old[w] = t;
-    if (m < thr) t = 1;
-    else if (m >= thr) t = 2;
-    type[w] = t;
-    unsigned long long hb = __ballot(t == 2);
-    if ((threadIdx.x & 63) == 0 && hb) {
-        atomicAdd(&stats[0], (unsigned long long)__popcll(hb));
-        atomicAdd(&stats[1], (unsigned long long)__popcll(hb));
+constexpr int CL_PT = 4;  // (group, tile) items per thread
+
+__global__ __launch_bounds__(1024) void classify_kernel(const InstDesc* inst, const int* tile_inst, int total_tiles,
+                                                        int ngroups, int broadcast, const double* md, double thr,
+                                                        int* type_old, int* type, unsigned long long* stats) {
+    __shared__ unsigned cnt[16];
+    unsigned nhi = 0;
+    const int n = ngroups * total_tiles;
+#pragma unroll
+    for (int k = 0; k < CL_PT; ++k) {
+        const int w = (blockIdx.x * CL_PT + k) * blockDim.x + threadIdx.x;
+        if (w >= n) continue;
+        const int g = w / total_tiles, gtile = w - g * total_tiles;
+        const InstDesc& d = inst[tile_inst[gtile]];
+        const int lt = gtile - d.tile_base, uy = lt / d.tx, ux = lt - uy * d.tx;
+        const double* m0 = md + (size_t)(broadcast ? 0 : g) * total_tiles;
+        double m = m0[gtile];
+        if (uy < d.ty - 1) m = fmax(m, m0[gtile + d.tx]);
+        if (uy > 0) m = fmax(m, m0[gtile - d.tx]);
+        if (ux < d.tx - 1) m = fmax(m, m0[gtile + 1]);
+        if (ux > 0) m = fmax(m, m0[gtile - 1]);
+        int t = type[w];
+        type_old[w] = t;
+        if (m < thr) t = 1;
+        else if (m >= thr) t = 2;
+        type[w] = t;
+        nhi += t == 2 ? 1u : 0u;
+    }
+    // FP64 count of the block: wave sums, then one atomic per counter per
+    // block (per-wave atomics on one address serialise)
+    const unsigned ws = (unsigned)__ockl_wfred_add_i32((int)nhi);
+    if ((threadIdx.x & 63) == 0) cnt[threadIdx.x >> 6] = ws;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tot = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) tot += cnt[i];
+        if (tot) {
+            atomicAdd(&stats[0], tot);
+            atomicAdd(&stats[1], tot);
+        }
+    }
+}
+
+// Clears same_flags[f][tile] where the type group of field f changed type on
+// the tile (the monitor pass converted it: its two buffers no longer agree).
+__global__ void clear_changed_kernel(const int* __restrict__ type_old, const int* __restrict__ type, GroupMap gm,
+                                     int nfields, int total_tiles, int* __restrict__ same_flags,
+                                     unsigned long long* __restrict__ zmask) {
+    const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (e >= (long long)nfields * total_tiles) return;
+    const int f = (int)(e / total_tiles), t = (int)(e - (long long)f * total_tiles);
+    const int g = gm.fg[f];
+    if (g < 0) return;
+    const size_t k = (size_t)g * total_tiles + t;
+    if (type_old[k] != type[k]) {
+        same_flags[e] = 0;
+        if (zmask) atomicAnd(zmask + t, ~(1ull << f));
     }
 }
 
@@ -568,8 +645,10 @@ int launch_init(const InstDesc* inst_dev, int inst, long long ncells, int model,
 
 int launch_convert(const InstDesc* inst, const int* tile_inst, const Buf& b, long long total_cells, int total_tiles,
                    const int* type_old, const int* type, const GroupMap& gm, int mode, int tile, hipStream_t s) {
-    int grid = gm.ngroups * total_tiles;
+    const int items = gm.ngroups * total_tiles;
+    int grid = (items + WG - 1) / WG;
     if (grid > 65536) grid = 65536;
+    if (grid < 1) grid = 1;
     if (tile == 16) hipLaunchKernelGGL((convert_kernel<16>), dim3(grid), dim3(256), 0, s, inst, tile_inst, b, total_cells, total_tiles, type_old, type, gm, mode);
     else if (tile == 32) hipLaunchKernelGGL((convert_kernel<32>), dim3(grid), dim3(256), 0, s, inst, tile_inst, b, total_cells, total_tiles, type_old, type, gm, mode);
     else if (tile == 64) hipLaunchKernelGGL((convert_kernel<64>), dim3(grid), dim3(256), 0, s, inst, tile_inst, b, total_cells, total_tiles, type_old, type, gm, mode);
@@ -771,8 +850,9 @@ int launch_monitor_last_init(const StepArgs& a, int tile, int ngroups_eval, int 
 int launch_classify(const InstDesc* inst, const int* tile_inst, int total_tiles, int ngroups, int broadcast,
                     const double* md, double thr, int* type_old, int* type, unsigned long long* stats, hipStream_t s) {
     const int n = ngroups * total_tiles;
-    hipLaunchKernelGGL(classify_kernel, dim3((n + 255) / 256), dim3(256), 0, s, inst, tile_inst, total_tiles, ngroups,
-                       broadcast, md, thr, type_old, type, stats);
+    const int per = 1024 * CL_PT;
+    hipLaunchKernelGGL(classify_kernel, dim3(n > 0 ? (n + per - 1) / per : 1), dim3(1024), 0, s, inst, tile_inst,
+                       total_tiles, ngroups, broadcast, md, thr, type_old, type, stats);
     return hip_ok();
 }
 
@@ -783,6 +863,15 @@ int launch_gather(const InstDesc* inst, const Buf& b, long long total_cells, int
     else if (tile == 32) hipLaunchKernelGGL((gather_kernel<32>), dim3(grid), dim3(256), 0, s, inst, b, total_cells, total_tiles, type, group, instid, field, dst);
     else if (tile == 64) hipLaunchKernelGGL((gather_kernel<64>), dim3(grid), dim3(256), 0, s, inst, b, total_cells, total_tiles, type, group, instid, field, dst);
     else return AMPS_E_UNSUPPORTED;
+    return hip_ok();
+}
+
+int launch_clear_changed(const int* type_old, const int* type, const GroupMap& gm, int nfields, int total_tiles,
+                         int* same_flags, unsigned long long* zmask, hipStream_t s) {
+    const long long n = (long long)nfields * total_tiles;
+    if (n <= 0) return AMPS_OK;
+    hipLaunchKernelGGL(clear_changed_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, type_old, type, gm,
+                       nfields, total_tiles, same_flags, zmask);
     return hip_ok();
 }
 

@@ -112,6 +112,24 @@ __device__ inline void gg_wave_load(GgRaw& r, const StepArgs& a, const InstDesc&
 
 __device__ inline float half_word(unsigned w) { return __half2float(__ushort_as_half((unsigned short)w)); }
 
+// Held field-tiles (adaptive step): a low field-tile whose tile code and the
+// codes of all 8 neighbour tiles bound every value of the field within
+// GG_FLAT of 0 (|off| + 2^14 scl <= GG_FLAT) is an absent grain, decaying
+// towards 0 by ~2.5% a step on the workload grids; it is set to exactly 0
+// (uniform: words 0, offset 0, scale 0), a change below GG_FLAT, and stays
+// there (0 is a fixed point of the update).  Buffer zmask bit f of a tile
+// records "field f is exactly 0 here": a field whose bit is set on all 9
+// tiles needs no neighbour codes at all, and the monitor's value of such a
+// field is exactly 0.  A field
+// bounded within GG_FLAT of 1 on the 9 tiles, on a tile where every other
+// field is held near 0, is a grain interior: eta = 1, S - eta^2 = 0 is a
+// fixed point of the update when alpha = beta, so it is held as well.
+// same_flags[f][tile] records that both buffers already hold it, so its
+// loads and stores are skipped; its share of S is then 0, or off^2 near 1
+// (|v^2 - off^2| <= 3 GG_FLAT).  The held set of a tile is decided once from
+// the neighbour tables; the next-field prefetch skips held fields.
+constexpr double GG_FLAT = 1.0e-10;
+
 // Read-only metadata (descriptors, tile -> instance map, tile types) arrive
 // as restrict-qualified arguments so that the per-tile reads go through the
 // scalar cache; with MULTI = false the descriptor is loop-invariant.
@@ -154,11 +172,21 @@ __global__ __launch_bounds__(64) void gg_wave_kernel(StepArgs a, const InstDesc*
                 hbx[k] = bx;
             }
         }
+        unsigned long long zall = 0ull;  // fields exactly 0 on all 9 tiles
         if (ADAPT) {
             Nb nb;
             nb_setup(nb, d, tile_x, tile_y, TS);
+            const unsigned long long zm = a.in.zmask[nb_tile(nb, lane < 9 ? lane : 8)];
+            zall = ~0ull;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                const unsigned lo = __builtin_amdgcn_readlane((unsigned)(zm & 0xFFFFFFFFull), k);
+                const unsigned hi = __builtin_amdgcn_readlane((unsigned)(zm >> 32), k);
+                zall &= ((unsigned long long)hi << 32) | lo;
+            }
             for (int e = lane; e < a.q * 9; e += GW) {
                 const int f = e / 9, k = e - f * 9;
+                if ((zall >> f) & 1ull) continue;  // no codes needed
                 const int g = nb_tile(nb, k);
                 L.ntype[f][k] = types[(size_t)f * a.total_tiles + g];
                 L.noff[f][k] = a.in.off[(size_t)f * a.total_tiles + g];
@@ -172,9 +200,77 @@ __global__ __launch_bounds__(64) void gg_wave_kernel(StepArgs a, const InstDesc*
             S[0] = u.x; S[1] = u.y; S[2] = v.x; S[3] = v.y;
         }
         __syncthreads();  // neighbour tables visible
+        // held fields (bit f) and those both buffers already hold
+        unsigned long long held = 0ull, skip = 0ull, samem = 0ull, near0 = 0ull, zout = 0ull;
+        if (ADAPT) {
+            bool h0 = lane < a.q, h1 = lane < a.q;
+            int sfl = 0;
+            if (lane < a.q) sfl = a.same_flags[(size_t)lane * a.total_tiles + w];
+            if (lane < a.q && ((zall >> lane) & 1ull)) {
+                h1 = false;  // zero on all 9 tiles (codes not loaded)
+            } else if (lane < a.q) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) {
+                    const bool low = L.ntype[lane][k] == 1;
+                    const double o = L.noff[lane][k], bnd = 16384.0 * (double)L.nscl[lane][k];
+                    h0 = h0 && low && fabs(o) + bnd <= GG_FLAT;
+                    h1 = h1 && low && fabs(1.0 - o) + bnd <= GG_FLAT;
+                }
+            }
+            near0 = __ballot(h0);
+            const unsigned long long near1 = __ballot(h1);
+            const unsigned long long all = (1ull << a.q) - 1ull;
+            held = near0;
+            // a field near 1 is held when every other field of the tile is held near 0
+            // (eta = 1 is the bulk value only when alpha = beta)
+            if (al == be && near1 && __popcll(near1) == 1 && (near0 | near1) == all && (near0 & near1) == 0ull)
+                held |= near1;
+            samem = __ballot(sfl != 0);
+            skip = held & samem;
+        }
+        const unsigned long long live = ~held;  // fields that are updated
+        const int f0 = (int)__builtin_ctzll(live | (1ull << 63));
         GgRaw raw;
-        gg_wave_load<ADAPT>(raw, a, d, 0, ADAPT ? L.ntype[0][4] : 2, cell0, hoff, hreg, nh, L);
+        if (f0 < a.q) gg_wave_load<ADAPT>(raw, a, d, f0, ADAPT ? L.ntype[f0][4] : 2, cell0, hoff, hreg, nh, L);
         for (int f = 0; f < a.q; ++f) {
+            if (ADAPT && ((held >> f) & 1ull) && ((near0 >> f) & 1ull)) {
+                // absent grain: exactly 0 from now on
+                zout |= 1ull << f;
+                if ((skip >> f) & 1ull) continue;  // both buffers hold the zeros
+                *reinterpret_cast<uint2*>(a.out.f16 + (size_t)f * a.total_cells + d.cell_base + cell0) = make_uint2(0u, 0u);
+                if (lane == 0) {
+                    const bool in_zero = ((zall >> f) & 1ull) || (L.noff[f][4] == 0.0 && L.nscl[f][4] == 0.0f);
+                    a.out.off[(size_t)f * a.total_tiles + w] = 0.0;
+                    a.out.scl[(size_t)f * a.total_tiles + w] = 0.0f;
+                    a.same_flags[(size_t)f * a.total_tiles + w] = in_zero ? 1 : 0;  // equal once the input is 0 too
+                }
+                continue;
+            }
+            if (ADAPT && ((held >> f) & 1ull)) {
+                const double off = L.noff[f][4];
+                if ((skip >> f) & 1ull) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) Sn[i] += off * off;
+                    continue;
+                }
+                // first held step: copy into the other buffer (words, codes)
+                const float sc = L.nscl[f][4];
+                const __half* in16 = a.in.f16 + (size_t)f * a.total_cells + d.cell_base;
+                const uint2 wv = *reinterpret_cast<const uint2*>(in16 + cell0);
+                *reinterpret_cast<uint2*>(a.out.f16 + (size_t)f * a.total_cells + d.cell_base + cell0) = wv;
+                const unsigned hv[4] = {wv.x & 0xFFFFu, wv.x >> 16, wv.y & 0xFFFFu, wv.y >> 16};
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const double v = off + (double)half_word(hv[i]) * (double)sc;
+                    Sn[i] += v * v;
+                }
+                if (lane == 0) {
+                    a.out.off[(size_t)f * a.total_tiles + w] = off;
+                    a.out.scl[(size_t)f * a.total_tiles + w] = sc;
+                    a.same_flags[(size_t)f * a.total_tiles + w] = 1;
+                }
+                continue;
+            }
             const int ty = ADAPT ? L.ntype[f][4] : 2;
             // stage field f (own strip + halo) into the LDS box
             if (ty == 2) {
@@ -209,7 +305,11 @@ __global__ __launch_bounds__(64) void gg_wave_kernel(StepArgs a, const InstDesc*
                     }
                 }
             }
-            if (f + 1 < a.q) gg_wave_load<ADAPT>(raw, a, d, f + 1, ADAPT ? L.ntype[f + 1][4] : 2, cell0, hoff, hreg, nh, L);
+            {
+                // next updated field's loads in flight while this one is computed
+                const int fn = f + 1 < a.q ? (int)__builtin_ctzll((live >> (f + 1) << (f + 1)) | (1ull << 63)) : a.q;
+                if (fn < a.q) gg_wave_load<ADAPT>(raw, a, d, fn, ADAPT ? L.ntype[fn][4] : 2, cell0, hoff, hreg, nh, L);
+            }
             __syncthreads();
             if (ty == 2) {
                 double ru[6], rm[6], rd[6];
@@ -294,13 +394,16 @@ __global__ __launch_bounds__(64) void gg_wave_kernel(StepArgs a, const InstDesc*
                 if (lane == 0) {
                     a.out.off[(size_t)f * a.total_tiles + w] = noff;
                     a.out.scl[(size_t)f * a.total_tiles + w] = tc.scl;
+                    if (ADAPT && ((samem >> f) & 1ull)) a.same_flags[(size_t)f * a.total_tiles + w] = 0;
                 }
+                if (tc.scl == 0.0f && noff == 0.0) zout |= 1ull << f;
             }
             __syncthreads();  // the box is rewritten for the next field
         }
         double* so = a.aux_out + d.cell_base + cell0;
         *reinterpret_cast<double2*>(so) = make_double2(Sn[0], Sn[1]);
         *reinterpret_cast<double2*>(so + 2) = make_double2(Sn[2], Sn[3]);
+        if (ADAPT && lane == 0) a.out.zmask[w] = zout;
     }
 }
 
@@ -482,13 +585,32 @@ template <int TS>
 __global__ __launch_bounds__(256) void gg_aux_kernel(StepArgs a, const int* __restrict__ type_old,
                                                      const int* __restrict__ types) {
     constexpr int PER = TS * TS / WG;
-    for (int w = blockIdx.x; w < a.total_tiles; w += gridDim.x) {
+    __shared__ int chg[WG], todo[WG];
+    __shared__ int ntodo;
+    // chunks of WG / q tiles: thread (tile j, field f) checks one type pair,
+    // the tiles with a changed field (every tile without type_old) are
+    // compacted and recomputed one by one
+    const int tpc = WG / a.q > 0 ? WG / a.q : 1;
+    for (int c0 = blockIdx.x * tpc; c0 < a.total_tiles; c0 += gridDim.x * tpc) {
+        if ((int)threadIdx.x < tpc) chg[threadIdx.x] = type_old ? 0 : 1;
+        __syncthreads();
         if (type_old) {
-            bool changed = false;
-            for (int f = 0; f < a.q; ++f)
-                changed = changed || type_old[(size_t)f * a.total_tiles + w] != types[(size_t)f * a.total_tiles + w];
-            if (!changed) continue;
+            const int j = (int)threadIdx.x / a.q, f = (int)threadIdx.x - j * a.q, w = c0 + j;
+            if (j < tpc && w < a.total_tiles &&
+                type_old[(size_t)f * a.total_tiles + w] != types[(size_t)f * a.total_tiles + w])
+                chg[j] = 1;  // every writer stores 1
         }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int k = 0;
+            for (int i = 0; i < tpc; ++i)
+                if (chg[i] && c0 + i < a.total_tiles) todo[k++] = c0 + i;
+            ntodo = k;
+        }
+        __syncthreads();
+        const int nt = ntodo;
+        for (int it = 0; it < nt; ++it) {
+        const int w = todo[it];
         const InstDesc& d = a.inst[a.tile_inst[w]];
         const int lt = w - d.tile_base;
         const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
@@ -519,6 +641,8 @@ __global__ __launch_bounds__(256) void gg_aux_kernel(StepArgs a, const int* __re
         }
 #pragma unroll
         for (int i = 0; i < PER; ++i) a.aux_in[gc[i]] = S[i];
+        }
+        __syncthreads();  // todo and ntodo are rewritten by the next chunk
     }
 }
 
@@ -551,7 +675,10 @@ int launch_gg_step(const StepArgs& a, int tile, bool adaptive, int grid, hipStre
 }
 
 int launch_gg_aux(const StepArgs& a, int tile, const int* type_old, hipStream_t s) {
-    int grid = a.total_tiles > 65536 ? 65536 : a.total_tiles;
+    const int tpc = WG / a.q > 0 ? WG / a.q : 1;
+    int grid = (a.total_tiles + tpc - 1) / tpc;
+    if (grid > 65536) grid = 65536;
+    if (grid < 1) grid = 1;
     if (tile == 16) hipLaunchKernelGGL((gg_aux_kernel<16>), dim3(grid), dim3(WG), 0, s, a, type_old, a.type);
     else if (tile == 32) hipLaunchKernelGGL((gg_aux_kernel<32>), dim3(grid), dim3(WG), 0, s, a, type_old, a.type);
     else return AMPS_E_UNSUPPORTED;

@@ -980,6 +980,10 @@ __global__ __launch_bounds__(256, 3) void sf_mixed_kernel(StepArgs a, const Inst
 #pragma unroll
     for (int k = 0; k < 8; ++k) { cnt[k] = fcnt[k]; nf += cnt[k]; }
     const int nh = *wlc, n = nh + nf, G = gridDim.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // tile counts of the step (one writer per launch)
+        a.stats[2] += (unsigned long long)nh;
+        a.stats[3] += (unsigned long long)nf;
+    }
     for (int i = blockIdx.x; i < n; i += G) {
         if (i < nh) {
             const int w = wl[xcd_remap(i, nh)];

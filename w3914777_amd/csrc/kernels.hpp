@@ -32,6 +32,10 @@ int launch_advance_step(long long* step_ctr, int n, hipStream_t s);
 
 // mid (may be null) is recorded between the FP64 and the low-precision sweep.
 int launch_sf_step(const StepArgs& a, int tile, bool adaptive, hipStream_t s, hipEvent_t mid = nullptr);
+// same_flags[f][tile] = 0 and bit f of zmask[tile] cleared for every field f whose
+// type group changed type on the tile.
+int launch_clear_changed(const int* type_old, const int* type, const GroupMap& gm, int nfields, int total_tiles,
+                         int* same_flags, unsigned long long* zmask, hipStream_t s);
 int launch_worklist(const int* type, const int* type_old, int total_tiles, int* wl_hi, int* wl_lo, int* counts,
                     int* same_flags, hipStream_t s);
 int launch_gg_step(const StepArgs& a, int tile, bool adaptive, int grid, hipStream_t s);

@@ -56,35 +56,37 @@ __device__ inline void sf_unit_power(R gx, R gy, int n, R& c, R& s) {
     }
 }
 
-// (cos 6 theta, sin 6 theta) for the single-precision path: the unit vector
-// z^2/|z|^2 = (cos 2 theta, sin 2 theta) of z = gx + i gy, cubed.  (gx, gy)
-// is first scaled exactly by a power of two to max(|u|,|v|) in [0.5, 1), so
-// the squares neither underflow nor overflow for any finite gradient,
-// subnormal ones included (a reciprocal of a subnormal maximum would
-// overflow).  A zero gradient gives (1, 0).
-__device__ inline void sf_unit_power6(float gx, float gy, float& c, float& s) {
-    const float mx = fmaxf(fabsf(gx), fabsf(gy));
-    const int e = __builtin_amdgcn_frexp_expf(mx);
-    const float u = __builtin_amdgcn_ldexpf(gx, -e), v = __builtin_amdgcn_ldexpf(gy, -e);
-    const float uu = u * u, vv = v * v, r2 = uu + vv;
-    const bool nz = r2 > 0.0f;
-    const float inv = __builtin_amdgcn_rcpf(nz ? r2 : 1.0f);
-    const float c2 = nz ? (uu - vv) * inv : 1.0f, s2 = nz ? 2.0f * u * v * inv : 0.0f;
-    const float cc = c2 * c2, ss = s2 * s2;
-    c = c2 * (cc - 3.0f * ss);
-    s = s2 * (3.0f * cc - ss);
+// (cos 6 theta, sin 6 theta): the unit vector z^2/|z|^2 = (cos 2 theta,
+// sin 2 theta) of z = gx + i gy, cubed.  (gx, gy) is first scaled exactly by
+// a power of two to max(|u|,|v|) in [0.5, 1), so the squares neither
+// underflow nor overflow for any finite gradient, subnormal ones included (a
+// reciprocal of a subnormal maximum would overflow).  A zero gradient gives
+// (1, 0), as atan2(0, 0) = 0 does.
+__device__ inline int frexp_exp(float x) { return __builtin_amdgcn_frexp_expf(x); }
+__device__ inline int frexp_exp(double x) { return __builtin_amdgcn_frexp_exp(x); }
+__device__ inline float ldexp_any(float x, int e) { return __builtin_amdgcn_ldexpf(x, e); }
+__device__ inline double ldexp_any(double x, int e) { return __builtin_amdgcn_ldexp(x, e); }
+
+template <typename R>
+__device__ inline void sf_unit_power6(R gx, R gy, R& c, R& s) {
+    const R mx = fmax(fabs(gx), fabs(gy));
+    const int e = frexp_exp(mx);
+    const R u = ldexp_any(gx, -e), v = ldexp_any(gy, -e);
+    const R uu = u * u, vv = v * v, r2 = uu + vv;
+    const bool nz = r2 > R(0);
+    const R inv = rcp_any(nz ? r2 : R(1));
+    const R c2 = nz ? (uu - vv) * inv : R(1), s2 = nz ? R(2) * u * v * inv : R(0);
+    const R cc = c2 * c2, ss = s2 * s2;
+    c = c2 * (cc - R(3) * ss);
+    s = s2 * (R(3) * cc - ss);
 }
 
 template <typename R>
 __device__ inline void sf_aniso(R gx, R gy, int n, R cth0, R sth0, R epsb, R epsb_delta, R m_epsb_n_delta,
                                 R& eps, R& epsd) {
     R c, s;
-    if constexpr (sizeof(R) == 4) {
-        if (n == 6) sf_unit_power6(gx, gy, c, s);
-        else sf_unit_power<R>(gx, gy, n, c, s);
-    } else {
-        sf_unit_power<R>(gx, gy, n, c, s);
-    }
+    if (n == 6) sf_unit_power6<R>(gx, gy, c, s);  // the snowflake's six-fold case
+    else sf_unit_power<R>(gx, gy, n, c, s);
     R ca = c * cth0 + s * sth0;  // cos(n (theta - theta0))
     R sa = s * cth0 - c * sth0;  // sin(n (theta - theta0))
     eps = epsb + epsb_delta * ca;
