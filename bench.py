@@ -52,7 +52,7 @@ def step_chunked(ctx, steps):
 def run_path(amps, cfg, warmup, steps, kernel_pass=True, with_fields=True):
     """Timed run (graph replay, wall clock and HIP events around the step
     loop), then -- for the roofline -- the same warmup + steps again from the
-    same initial state with HIP events around every sweep kernel (graph
+    same initial state with HIP events bound to every sweep kernel (graph
     replay suspended), so the per-kernel averages cover the same trajectory
     as the timed region."""
     with amps.Context(cfg) as ctx:
@@ -229,6 +229,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-others", action="store_true", help="skip the grain-growth, sintering and ensemble lines")
+    ap.add_argument("--traffic-run", action="store_true",
+                    help="one adaptive run of warmup + steps only, printing the mixed sweep's algorithmic bytes "
+                         "per launch over it (the companion of a rocprofv3 --pmc pass)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported (the path is single-GPU).",
@@ -248,6 +251,17 @@ def main():
 
     if args.copy_probe:
         copy_probe()
+    if args.traffic_run:
+        log(f"[bench] traffic run: adaptive {cfg.nx}x{cfg.ny}, {args.warmup}+{args.steps} steps")
+        r = run_path(amps, amp_cfg, args.warmup, args.steps, kernel_pass=False, with_fields=False)
+        n = args.warmup + args.steps   # one mixed-sweep launch per step
+        i = r["info"]
+        tc = cfg.tile * cfg.tile
+        print(json.dumps({"traffic_run": "sf_mixed_kernel<32,false>", "launches": n,
+                          "fp64_tile_updates": i.fp64_tile_updates, "low_full_tile_updates": i.low_full_tile_updates,
+                          "alg_bytes_per_launch": (i.fp64_tile_updates * tc * B_FP64
+                                                   + i.low_full_tile_updates * tc * B_FP16) / n}))
+        return
     log(f"[bench] adaptive path: {cfg.nx}x{cfg.ny}, warmup {args.warmup}, steps {args.steps}")
     amp = run_path(amps, amp_cfg, args.warmup, args.steps)
     log(f"[bench] adaptive: {amp['wall']:.3f} s wall, {amp['ev_ms']:.1f} ms events, hi_frac {amp['info'].hi_frac_mean:.4f}")
@@ -289,8 +303,8 @@ def main():
                        "low_full_tiles_per_launch": full_tiles}
     out["adaptive_step_kernels_us"] = {"sf_mixed_kernel<32,false>": mix_s * 1e6,
                                        "sf_hold_kernel<32,1,false>": hold_s * 1e6,
-                                       "note": "per-launch averages over the timed steps, HIP events on the "
-                                               "context stream (include the dispatch gap)"}
+                                       "note": "per-launch averages over the timed steps, HIP events bound "
+                                               "to each kernel's dispatch (its execution time)"}
     out["hi_frac_mean"] = amp["info"].hi_frac_mean
     if f64 is not None:
         f64_value = cells * args.steps / f64["wall"]

@@ -220,9 +220,10 @@ int amps_get_info(amps_ctx* ctx, amps_info* info);
 
 /* ---- measurement ---- */
 
-/* When enabled, every sweep kernel launched by amps_step is bracketed by HIP
- * events on the context's stream (graph replay is suspended meanwhile), and
- * the counters below restart from zero.  Replaces the per-kernel cudaEvent
+/* When enabled, every sweep kernel launched by amps_step carries a pair of
+ * HIP events bound to its own dispatch (hipExtLaunchKernel), so the summed
+ * durations are the kernels' execution times without dispatch gaps; graph
+ * replay is suspended meanwhile, and the counters below restart from zero.  Replaces the per-kernel cudaEvent
  * pairs of the reference's timing-decomposition drivers
  * (snowflake/monitor_independent_withoutmonitor.cu:85-127). */
 int amps_set_kernel_timing(amps_ctx* ctx, int enable);

@@ -114,7 +114,7 @@ struct amps_ctx {
 
     // sweep-kernel timing (events around each step-kernel launch)
     bool ktiming = false;
-    std::vector<hipEvent_t> kev;  // triples: before, between the two snowflake sweeps, after
+    std::vector<hipEvent_t> kev;  // quads: start and end of a step's first and second sweep kernel
     int kev_used = 0;
     long long klaunches = 0;
     double kms = 0.0;                      // all sweep kernels
@@ -277,8 +277,8 @@ int kev_flush(amps_ctx* c) {
     HIPCHK(hipStreamSynchronize(c->stream));
     for (int i = 0; i < c->kev_used; ++i) {
         float a = 0.0f, b = 0.0f;
-        HIPCHK(hipEventElapsedTime(&a, c->kev[3 * i], c->kev[3 * i + 1]));
-        HIPCHK(hipEventElapsedTime(&b, c->kev[3 * i + 1], c->kev[3 * i + 2]));
+        HIPCHK(hipEventElapsedTime(&a, c->kev[4 * i], c->kev[4 * i + 1]));
+        HIPCHK(hipEventElapsedTime(&b, c->kev[4 * i + 2], c->kev[4 * i + 3]));
         c->kms += (double)a + (double)b;
         c->kms_part[0] += a;
         c->kms_part[1] += b;
@@ -288,28 +288,23 @@ int kev_flush(amps_ctx* c) {
     return AMPS_OK;
 }
 
-int launch_step_kernel(amps_ctx* c, const StepArgs& a, hipEvent_t mid = nullptr) {
+int launch_step_kernel(amps_ctx* c, const StepArgs& a, const hipEvent_t* ev = nullptr) {
     const bool adaptive = has_low(c);
     const int grid = a.grid_cap > 0 && a.grid_cap < a.nwork ? a.grid_cap : a.nwork;
-    if (c->model == AMPS_SNOWFLAKE) return launch_sf_step(a, c->tile, adaptive, c->stream, mid);
-    int rc = c->model == AMPS_GRAINGROWTH ? launch_gg_step(a, c->tile, adaptive, grid, c->stream)
-                                          : launch_sn_step(a, c->tile, adaptive, grid, c->stream);
-    if (rc == AMPS_OK && mid) HIPCHK(hipEventRecord(mid, c->stream));
-    return rc;
+    if (c->model == AMPS_SNOWFLAKE) return launch_sf_step(a, c->tile, adaptive, c->stream, ev);
+    return c->model == AMPS_GRAINGROWTH ? launch_gg_step(a, c->tile, adaptive, grid, c->stream, ev)
+                                        : launch_sn_step(a, c->tile, adaptive, grid, c->stream, ev);
 }
 
 // Enqueue one explicit-Euler step reading buffer `from`.  With kernel timing
-// on, events bracket the sweep kernels (and separate the snowflake's FP64
-// and low-precision sweeps).
+// on, each sweep kernel carries a pair of events bound to its own dispatch
+// (start and end of its execution).
 int enqueue_step(amps_ctx* c, int from, int step_add, bool capturing = false) {
     StepArgs a = step_args(c, from);
     a.step_add = step_add;
     if (!c->ktiming || capturing) return launch_step_kernel(c, a);
-    if (c->kev_used * 3 + 3 > (int)c->kev.size()) RCCHK(kev_flush(c));
-    hipEvent_t* e = &c->kev[3 * c->kev_used];
-    HIPCHK(hipEventRecord(e[0], c->stream));
-    RCCHK(launch_step_kernel(c, a, e[1]));
-    HIPCHK(hipEventRecord(e[2], c->stream));
+    if (c->kev_used * 4 + 4 > (int)c->kev.size()) RCCHK(kev_flush(c));
+    RCCHK(launch_step_kernel(c, a, &c->kev[4 * c->kev_used]));
     c->kev_used++;
     return AMPS_OK;
 }
@@ -847,7 +842,7 @@ extern "C" int amps_set_kernel_timing(amps_ctx* c, int enable) {
     if (!c) return fail(AMPS_E_ARG, "ctx is NULL");
     RCCHK(kev_flush(c));
     if (enable && c->kev.empty()) {
-        c->kev.resize(3 * 512);
+        c->kev.resize(4 * 512);
         for (auto& e : c->kev) HIPCHK(hipEventCreate(&e));
     }
     c->ktiming = enable != 0;

@@ -647,27 +647,28 @@ __global__ __launch_bounds__(256) void gg_aux_kernel(StepArgs a, const int* __re
 }
 
 template <int TS, bool ADAPT>
-static void gg_launch(const StepArgs& a, int grid, hipStream_t s) {
+static void gg_launch(const StepArgs& a, int grid, hipStream_t s, const hipEvent_t* ev) {
     if constexpr (TS == 16) {
         if (a.ninst > 1)
-            hipLaunchKernelGGL((gg_wave_kernel<TS, ADAPT, true>), dim3(grid), dim3(GW), 0, s, a, a.inst, a.tile_inst, a.type);
+            launch_sweep(gg_wave_kernel<TS, ADAPT, true>, dim3(grid), dim3(GW), s, ev, a, a.inst, a.tile_inst, a.type);
         else
-            hipLaunchKernelGGL((gg_wave_kernel<TS, ADAPT, false>), dim3(grid), dim3(GW), 0, s, a, a.inst, a.tile_inst, a.type);
+            launch_sweep(gg_wave_kernel<TS, ADAPT, false>, dim3(grid), dim3(GW), s, ev, a, a.inst, a.tile_inst, a.type);
     } else {
         if (a.ninst > 1)
-            hipLaunchKernelGGL((gg_block_kernel<TS, ADAPT, true>), dim3(grid), dim3(WG), 0, s, a, a.inst, a.tile_inst, a.type);
+            launch_sweep(gg_block_kernel<TS, ADAPT, true>, dim3(grid), dim3(WG), s, ev, a, a.inst, a.tile_inst, a.type);
         else
-            hipLaunchKernelGGL((gg_block_kernel<TS, ADAPT, false>), dim3(grid), dim3(WG), 0, s, a, a.inst, a.tile_inst, a.type);
+            launch_sweep(gg_block_kernel<TS, ADAPT, false>, dim3(grid), dim3(WG), s, ev, a, a.inst, a.tile_inst, a.type);
     }
+    record_empty_pair(s, ev ? ev + 2 : nullptr);
 }
 
-int launch_gg_step(const StepArgs& a, int tile, bool adaptive, int grid, hipStream_t s) {
+int launch_gg_step(const StepArgs& a, int tile, bool adaptive, int grid, hipStream_t s, const hipEvent_t* ev) {
     if (tile == 16) {
-        if (adaptive) gg_launch<16, true>(a, grid, s);
-        else gg_launch<16, false>(a, grid, s);
+        if (adaptive) gg_launch<16, true>(a, grid, s, ev);
+        else gg_launch<16, false>(a, grid, s, ev);
     } else if (tile == 32) {
-        if (adaptive) gg_launch<32, true>(a, grid, s);
-        else gg_launch<32, false>(a, grid, s);
+        if (adaptive) gg_launch<32, true>(a, grid, s, ev);
+        else gg_launch<32, false>(a, grid, s, ev);
     } else {
         return AMPS_E_UNSUPPORTED;
     }

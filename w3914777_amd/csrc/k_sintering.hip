@@ -201,20 +201,21 @@ __global__ __launch_bounds__(256) void sn_step_kernel(StepArgs a, const InstDesc
 }
 
 template <int TS, bool ADAPT>
-static void sn_launch(const StepArgs& a, int grid, hipStream_t s) {
+static void sn_launch(const StepArgs& a, int grid, hipStream_t s, const hipEvent_t* ev) {
     if (a.ninst > 1)
-        hipLaunchKernelGGL((sn_step_kernel<TS, ADAPT, true>), dim3(grid), dim3(WG), 0, s, a, a.inst, a.tile_inst, a.type);
+        launch_sweep(sn_step_kernel<TS, ADAPT, true>, dim3(grid), dim3(WG), s, ev, a, a.inst, a.tile_inst, a.type);
     else
-        hipLaunchKernelGGL((sn_step_kernel<TS, ADAPT, false>), dim3(grid), dim3(WG), 0, s, a, a.inst, a.tile_inst, a.type);
+        launch_sweep(sn_step_kernel<TS, ADAPT, false>, dim3(grid), dim3(WG), s, ev, a, a.inst, a.tile_inst, a.type);
+    record_empty_pair(s, ev ? ev + 2 : nullptr);
 }
 
-int launch_sn_step(const StepArgs& a, int tile, bool adaptive, int grid, hipStream_t s) {
+int launch_sn_step(const StepArgs& a, int tile, bool adaptive, int grid, hipStream_t s, const hipEvent_t* ev) {
     if (tile == 16) {
-        if (adaptive) sn_launch<16, true>(a, grid, s);
-        else sn_launch<16, false>(a, grid, s);
+        if (adaptive) sn_launch<16, true>(a, grid, s, ev);
+        else sn_launch<16, false>(a, grid, s, ev);
     } else if (tile == 32) {
-        if (adaptive) sn_launch<32, true>(a, grid, s);
-        else sn_launch<32, false>(a, grid, s);
+        if (adaptive) sn_launch<32, true>(a, grid, s, ev);
+        else sn_launch<32, false>(a, grid, s, ev);
     } else {
         return AMPS_E_UNSUPPORTED;
     }

@@ -1035,7 +1035,7 @@ static int resident_grid(int nwork, int cap) {
 }
 
 template <int TS, bool MULTI>
-static void sf_launch(const StepArgs& a, bool adaptive, hipStream_t s, hipEvent_t mid) {
+static void sf_launch(const StepArgs& a, bool adaptive, hipStream_t s, const hipEvent_t* ev) {
     dim3 blk(WG);
     if (adaptive) {
         // bulk-hold sweep first (it lists the low tiles it does not hold),
@@ -1043,32 +1043,31 @@ static void sf_launch(const StepArgs& a, bool adaptive, hipStream_t s, hipEvent_
         constexpr auto km = sf_mixed_kernel<TS, MULTI>;
         if (a.t_halo == 2) {
             constexpr auto kb = sf_hold_kernel<TS, 2, MULTI>;
-            hipLaunchKernelGGL(kb, dim3(resident_grid<kb, 64>(a.total_tiles, 0)), dim3(64), 0, s, a, a.inst, a.tile_inst,
-                               a.wl_low, a.wl_low_count);
+            launch_sweep(kb, dim3(resident_grid<kb, 64>(a.total_tiles, 0)), dim3(64), s, ev, a, a.inst, a.tile_inst,
+                         a.wl_low, a.wl_low_count);
         } else {
             constexpr auto kb = sf_hold_kernel<TS, 1, MULTI>;
-            hipLaunchKernelGGL(kb, dim3(resident_grid<kb, 64>(a.total_tiles, 0)), dim3(64), 0, s, a, a.inst, a.tile_inst,
-                               a.wl_low, a.wl_low_count);
+            launch_sweep(kb, dim3(resident_grid<kb, 64>(a.total_tiles, 0)), dim3(64), s, ev, a, a.inst, a.tile_inst,
+                         a.wl_low, a.wl_low_count);
         }
-        if (mid) (void)hipEventRecord(mid, s);
-        hipLaunchKernelGGL(km, dim3(resident_grid<km>(a.total_tiles, a.grid_cap)), blk, 0, s, a, a.inst, a.tile_inst,
-                           a.worklist, a.wl_count, (const int*)a.full_seg, (const int*)a.full_cnt);
+        launch_sweep(km, dim3(resident_grid<km>(a.total_tiles, a.grid_cap)), blk, s, ev ? ev + 2 : nullptr, a, a.inst,
+                     a.tile_inst, a.worklist, a.wl_count, (const int*)a.full_seg, (const int*)a.full_cnt);
     } else {
         constexpr auto kh = sf_high_kernel<TS, false, MULTI>;
-        hipLaunchKernelGGL(kh, dim3(resident_grid<kh>(a.nwork, a.grid_cap)), blk, 0, s, a, a.inst, a.tile_inst,
-                           (const int*)nullptr, (const int*)nullptr);
-        if (mid) (void)hipEventRecord(mid, s);
+        launch_sweep(kh, dim3(resident_grid<kh>(a.nwork, a.grid_cap)), blk, s, ev, a, a.inst, a.tile_inst,
+                     (const int*)nullptr, (const int*)nullptr);
+        record_empty_pair(s, ev ? ev + 2 : nullptr);
     }
 }
 
-int launch_sf_step(const StepArgs& a, int tile, bool adaptive, hipStream_t s, hipEvent_t mid) {
+int launch_sf_step(const StepArgs& a, int tile, bool adaptive, hipStream_t s, const hipEvent_t* ev) {
     const bool multi = a.ninst > 1;
     if (tile == 16) {
-        if (multi) sf_launch<16, true>(a, adaptive, s, mid);
-        else sf_launch<16, false>(a, adaptive, s, mid);
+        if (multi) sf_launch<16, true>(a, adaptive, s, ev);
+        else sf_launch<16, false>(a, adaptive, s, ev);
     } else if (tile == 32) {
-        if (multi) sf_launch<32, true>(a, adaptive, s, mid);
-        else sf_launch<32, false>(a, adaptive, s, mid);
+        if (multi) sf_launch<32, true>(a, adaptive, s, ev);
+        else sf_launch<32, false>(a, adaptive, s, ev);
     } else {
         return AMPS_E_UNSUPPORTED;
     }

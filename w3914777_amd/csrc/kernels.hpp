@@ -30,18 +30,21 @@ int launch_nonfinite(const double* v, long long n, unsigned long long* cnt, hipS
 int launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t s);
 int launch_advance_step(long long* step_ctr, int n, hipStream_t s);
 
-// mid (may be null) is recorded between the FP64 and the low-precision sweep.
-int launch_sf_step(const StepArgs& a, int tile, bool adaptive, hipStream_t s, hipEvent_t mid = nullptr);
+// ev (may be null): four timing events, the start and end of the step's first
+// sweep kernel and of its second (snowflake adaptive: the bulk-hold sweep,
+// then the mixed sweep); a step with one sweep kernel records the second pair
+// back to back.
+int launch_sf_step(const StepArgs& a, int tile, bool adaptive, hipStream_t s, const hipEvent_t* ev = nullptr);
 // same_flags[f][tile] = 0 and bit f of zmask[tile] cleared for every field f whose
 // type group changed type on the tile.
 int launch_clear_changed(const int* type_old, const int* type, const GroupMap& gm, int nfields, int total_tiles,
                          int* same_flags, unsigned long long* zmask, hipStream_t s);
 int launch_worklist(const int* type, const int* type_old, int total_tiles, int* wl_hi, int* wl_lo, int* counts,
                     int* same_flags, hipStream_t s);
-int launch_gg_step(const StepArgs& a, int tile, bool adaptive, int grid, hipStream_t s);
+int launch_gg_step(const StepArgs& a, int tile, bool adaptive, int grid, hipStream_t s, const hipEvent_t* ev = nullptr);
 // type_old (may be null): refresh only tiles where some field's type differs from a.type.
 int launch_gg_aux(const StepArgs& a, int tile, const int* type_old, hipStream_t s);
-int launch_sn_step(const StepArgs& a, int tile, bool adaptive, int grid, hipStream_t s);
+int launch_sn_step(const StepArgs& a, int tile, bool adaptive, int grid, hipStream_t s, const hipEvent_t* ev = nullptr);
 
 // Reverse-time adjoint of one FP64 step (snowflake, sintering).  Arrays are
 // [field][total_cells] like the FP64 state; work arrays hold total_cells.

@@ -10,11 +10,29 @@
 //   low tile : off[tile] + f16[cell] * scl[tile]
 #pragma once
 
+#include <hip/hip_ext.h>
+
 #include "amps_internal.hpp"
 
 namespace amps {
 
 constexpr int WG = 256;
+
+// Launch a sweep kernel.  With ev (two timing events), hipExtLaunchKernel
+// binds both events to the dispatch itself, so their elapsed time is the
+// kernel's own execution time (no dispatch gap); without, a plain launch.
+template <typename K, typename... A>
+inline void launch_sweep(K k, dim3 grid, dim3 blk, hipStream_t s, const hipEvent_t* ev, A... args) {
+    if (ev) hipExtLaunchKernelGGL(k, grid, blk, 0, s, ev[0], ev[1], 0, args...);
+    else hipLaunchKernelGGL(k, grid, blk, 0, s, args...);
+}
+
+// Second timing pair of a step with one sweep kernel: recorded back to back.
+inline void record_empty_pair(hipStream_t s, const hipEvent_t* ev) {
+    if (!ev) return;
+    (void)hipEventRecord(ev[0], s);
+    (void)hipEventRecord(ev[1], s);
+}
 
 // Type of (tile, group); group < 0 means the field is always FP64.
 __device__ inline int tile_type(const StepArgs& a, int group, int gtile) {
