@@ -312,7 +312,8 @@ __global__ __launch_bounds__(256) void monitor_grad_kernel(StepArgs a, int ngrou
 // is read once.  Values are decoded and the quotient evaluated exactly as
 // above, so the monitor values are identical.
 struct MonWaveLds {
-    double box[18 * 20];
+    static constexpr int LWD = 22, C0 = 2;  // row stride (even, for 16-byte reads; 11 16-byte groups) and first own column
+    double box[18 * LWD];
     int ntype[AMPS_MAX_FIELDS][9];
     double noff[AMPS_MAX_FIELDS][9];
     float nscl[AMPS_MAX_FIELDS][9];
@@ -322,7 +323,8 @@ __global__ __launch_bounds__(64) void monitor_grad_wave16(StepArgs a, const Inst
                                                           const int* __restrict__ tinst,
                                                           const int* __restrict__ types, int ngroups_eval, int field0,
                                                           int field_stride, int type_groups, double* __restrict__ md) {
-    constexpr int TS = 16, LWD = 20, C0 = 2;
+    constexpr int TS = 16, LWD = MonWaveLds::LWD, C0 = MonWaveLds::C0;
+    static_assert(LWD % 2 == 0 && LWD >= C0 + TS + 2, "16-byte reads of columns x0-2 .. x0+5");
     __shared__ __align__(16) MonWaveLds L;
     const int lane = threadIdx.x;
     const int ly = lane >> 2, x0 = (lane & 3) << 2;
@@ -372,14 +374,40 @@ __global__ __launch_bounds__(64) void monitor_grad_wave16(StepArgs a, const Inst
         if (!uni) {
             Nb nb;
             nb_setup(nb, d, txi, tyi, TS);
-            for (int e = lane; e < ngroups_eval * 9; e += 64) {
-                const int g = e / 9, k = e - g * 9;
-                if ((z5 >> g) & 1ull) continue;  // value already written
-                const int field = field0 + g * field_stride, tgroup = type_groups > 1 ? g : 0;
-                const int gt = nb_tile(nb, k);
-                L.ntype[g][k] = types[(size_t)tgroup * a.total_tiles + gt];
-                L.noff[g][k] = a.in.off[(size_t)field * a.total_tiles + gt];
-                L.nscl[g][k] = a.in.scl[(size_t)field * a.total_tiles + gt];
+            // the loads of NB entries per lane in flight together, then their
+            // LDS stores (fields whose value is already written are skipped)
+            constexpr int NE = (AMPS_MAX_FIELDS * 9 + 63) / 64, NB = 3;
+            const unsigned ge = (unsigned)ngroups_eval * 9u;
+#pragma unroll 1
+            for (int i0 = 0; i0 < NE; i0 += NB) {
+                int tv[NB];
+                double ov[NB];
+                float sv[NB];
+                unsigned ent[NB];
+#pragma unroll
+                for (int i = 0; i < NB; ++i) {
+                    const unsigned e = (unsigned)(lane + (i0 + i) * 64), g = e / 9u, k = e - g * 9u;
+                    ent[i] = e < ge && !((z5 >> g) & 1ull) ? e : ~0u;
+                    tv[i] = 0;
+                    ov[i] = 0.0;
+                    sv[i] = 0.0f;
+                    if (ent[i] != ~0u) {
+                        const unsigned field = (unsigned)(field0 + (int)g * field_stride);
+                        const unsigned tgroup = type_groups > 1 ? g : 0u;
+                        const unsigned gt = (unsigned)nb_tile(nb, (int)k);
+                        tv[i] = types[tgroup * (unsigned)a.total_tiles + gt];
+                        ov[i] = a.in.off[field * (unsigned)a.total_tiles + gt];
+                        sv[i] = a.in.scl[field * (unsigned)a.total_tiles + gt];
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < NB; ++i) {
+                    if (ent[i] != ~0u) {
+                        (&L.ntype[0][0])[ent[i]] = tv[i];
+                        (&L.noff[0][0])[ent[i]] = ov[i];
+                        (&L.nscl[0][0])[ent[i]] = sv[i];
+                    }
+                }
             }
         }
         __syncthreads();
@@ -451,11 +479,22 @@ __global__ __launch_bounds__(64) void monitor_grad_wave16(StepArgs a, const Inst
             double m = 0.0;
             {
 #pragma clang fp contract(off)
+                // 16-byte reads: columns x0-2 .. x0+5 of the row, x0 .. x0+3 above and below
                 const double* c = L.box + (ly + 1) * LWD + C0 + x0;
+                const double2 m0 = *reinterpret_cast<const double2*>(c - 2);
+                const double2 m1 = *reinterpret_cast<const double2*>(c);
+                const double2 m2 = *reinterpret_cast<const double2*>(c + 2);
+                const double2 m3 = *reinterpret_cast<const double2*>(c + 4);
+                const double2 u0 = *reinterpret_cast<const double2*>(c - LWD);
+                const double2 u1 = *reinterpret_cast<const double2*>(c - LWD + 2);
+                const double2 d0 = *reinterpret_cast<const double2*>(c + LWD);
+                const double2 d1 = *reinterpret_cast<const double2*>(c + LWD + 2);
+                const double row[6] = {m0.y, m1.x, m1.y, m2.x, m2.y, m3.x};
+                const double up[4] = {u0.x, u0.y, u1.x, u1.y}, dn[4] = {d0.x, d0.y, d1.x, d1.y};
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const double hx = 0.5 * (c[i + 1] - c[i - 1]);
-                    const double hy = 0.5 * (c[i + LWD] - c[i - LWD]) * r;
+                    const double hx = 0.5 * (row[i + 2] - row[i]);
+                    const double hy = 0.5 * (dn[i] - up[i]) * r;
                     const double qv = hx * hx + hy * hy;
                     if (qv > m) m = qv;
                 }

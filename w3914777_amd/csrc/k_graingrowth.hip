@@ -172,6 +172,16 @@ __global__ __launch_bounds__(64) void gg_wave_kernel(StepArgs a, const InstDesc*
                 hbx[k] = bx;
             }
         }
+        // carried sum of squares and the same-data flags: independent of the
+        // tables below, so their loads go first
+        double S[4], Sn[4] = {0.0, 0.0, 0.0, 0.0};
+        {
+            const double2 u = *reinterpret_cast<const double2*>(a.aux_in + d.cell_base + cell0);
+            const double2 v = *reinterpret_cast<const double2*>(a.aux_in + d.cell_base + cell0 + 2);
+            S[0] = u.x; S[1] = u.y; S[2] = v.x; S[3] = v.y;
+        }
+        int sfl = 0;
+        if (ADAPT && lane < a.q) sfl = a.same_flags[(size_t)lane * a.total_tiles + w];
         unsigned long long zall = 0ull;  // fields exactly 0 on all 9 tiles
         if (ADAPT) {
             Nb nb;
@@ -184,28 +194,45 @@ __global__ __launch_bounds__(64) void gg_wave_kernel(StepArgs a, const InstDesc*
                 const unsigned hi = __builtin_amdgcn_readlane((unsigned)(zm >> 32), k);
                 zall &= ((unsigned long long)hi << 32) | lo;
             }
-            for (int e = lane; e < a.q * 9; e += GW) {
-                const int f = e / 9, k = e - f * 9;
-                if ((zall >> f) & 1ull) continue;  // no codes needed
-                const int g = nb_tile(nb, k);
-                L.ntype[f][k] = types[(size_t)f * a.total_tiles + g];
-                L.noff[f][k] = a.in.off[(size_t)f * a.total_tiles + g];
-                L.nscl[f][k] = a.in.scl[(size_t)f * a.total_tiles + g];
+            // neighbour tables of the fields not zero on all 9 tiles: the loads
+            // of NB entries per lane in flight together, then their LDS stores
+            constexpr int NE = (AMPS_MAX_FIELDS * 9 + GW - 1) / GW, NB = 3;
+            const unsigned qe = (unsigned)a.q * 9u;
+#pragma unroll 1
+            for (int i0 = 0; i0 < NE; i0 += NB) {
+                int tv[NB];
+                double ov[NB];
+                float sv[NB];
+                unsigned ent[NB];
+#pragma unroll
+                for (int i = 0; i < NB; ++i) {
+                    const unsigned e = (unsigned)(lane + (i0 + i) * GW), f = e / 9u, k = e - f * 9u;
+                    ent[i] = e < qe && !((zall >> f) & 1ull) ? f * 9u + k : ~0u;
+                    tv[i] = 0;
+                    ov[i] = 0.0;
+                    sv[i] = 0.0f;
+                    if (ent[i] != ~0u) {
+                        const unsigned idx = f * (unsigned)a.total_tiles + (unsigned)nb_tile(nb, (int)k);
+                        tv[i] = types[idx];
+                        ov[i] = a.in.off[idx];
+                        sv[i] = a.in.scl[idx];
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < NB; ++i) {
+                    if (ent[i] != ~0u) {
+                        (&L.ntype[0][0])[ent[i]] = tv[i];
+                        (&L.noff[0][0])[ent[i]] = ov[i];
+                        (&L.nscl[0][0])[ent[i]] = sv[i];
+                    }
+                }
             }
-        }
-        double S[4], Sn[4] = {0.0, 0.0, 0.0, 0.0};
-        {
-            const double2 u = *reinterpret_cast<const double2*>(a.aux_in + d.cell_base + cell0);
-            const double2 v = *reinterpret_cast<const double2*>(a.aux_in + d.cell_base + cell0 + 2);
-            S[0] = u.x; S[1] = u.y; S[2] = v.x; S[3] = v.y;
         }
         __syncthreads();  // neighbour tables visible
         // held fields (bit f) and those both buffers already hold
         unsigned long long held = 0ull, skip = 0ull, samem = 0ull, near0 = 0ull, zout = 0ull;
         if (ADAPT) {
             bool h0 = lane < a.q, h1 = lane < a.q;
-            int sfl = 0;
-            if (lane < a.q) sfl = a.same_flags[(size_t)lane * a.total_tiles + w];
             if (lane < a.q && ((zall >> lane) & 1ull)) {
                 h1 = false;  // zero on all 9 tiles (codes not loaded)
             } else if (lane < a.q) {
@@ -363,7 +390,17 @@ __global__ __launch_bounds__(64) void gg_wave_kernel(StepArgs a, const InstDesc*
 #undef GG_ROW6F
                 }
                 float lapv[4], rn[4];
-                gg_lap_row4<float>(ru, rm, rd, moore, invf, lapv);
+                if (moore) {
+                    // column sums shared by the neighbouring cells (the low path's
+                    // own operand order: sum of the 3x3 block minus 9 eta)
+                    float cs[6];
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) cs[j] = ru[j] + rm[j] + rd[j];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) lapv[i] = (cs[i] + cs[i + 1] + cs[i + 2] - 9.0f * rm[i + 1]) * invf;
+                } else {
+                    gg_lap_row4<float>(ru, rm, rd, moore, invf, lapv);
+                }
                 float mn = INFINITY, mx = -INFINITY;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
@@ -376,8 +413,7 @@ __global__ __launch_bounds__(64) void gg_wave_kernel(StepArgs a, const InstDesc*
                     mn = fminf(mn, rn[i]);
                     mx = fmaxf(mx, rn[i]);
                 }
-                mn = wave_min(mn);
-                mx = wave_max(mx);
+                wave_minmax(mn, mx);
                 const TileCode tc = tile_code(mn, mx);
                 const double noff = own_off + (double)tc.mid;
                 unsigned short hw[4];

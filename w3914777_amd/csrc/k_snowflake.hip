@@ -803,8 +803,7 @@ __device__ void sf_hold_tile(float* Tb, const StepArgs& a, const InstDesc& d, in
             mx = fmaxf(mx, tc[j]);
         }
     }
-    mn = wave_min(mn);
-    mx = wave_max(mx);
+    wave_minmax(mn, mx);
     const TileCode ct = tile_code(mn, mx);
     unsigned nw[R / 2];
 #pragma unroll

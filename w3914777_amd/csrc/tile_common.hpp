@@ -220,20 +220,47 @@ __device__ inline double rcp_nr(double x) {
 }
 
 // ----------------------------------------------------------- reductions
-// Wave-wide reductions (DPP, result in every lane) from the device library.
-extern "C" __device__ float __ockl_wfred_min_f32(float);
-extern "C" __device__ float __ockl_wfred_max_f32(float);
+// Wave-wide maximum of doubles (DPP, result in every lane) from the device library.
 extern "C" __device__ double __ockl_wfred_max_f64(double);
 
-__device__ inline float wave_min(float v) { return __ockl_wfred_min_f32(v); }
-__device__ inline float wave_max(float v) { return __ockl_wfred_max_f32(v); }
 __device__ inline double wave_max_d(double v) { return __ockl_wfred_max_f64(v); }
+
+// Order-preserving map between float bit patterns and signed integers (an
+// involution): non-negative floats keep their bits, negative ones have their
+// magnitude bits flipped, so integer order is float order (-0 < +0).
+__device__ inline int f32_key(float v) {
+    const int b = __float_as_int(v);
+    return b ^ ((b >> 31) & 0x7FFFFFFF);
+}
+__device__ inline float key_f32(int k) { return __int_as_float(k ^ ((k >> 31) & 0x7FFFFFFF)); }
+
+// Min and max of non-NaN floats over a full wave (all 64 lanes active),
+// returned in every lane.  Integer keys let each step of the row_shr DPP
+// ladder be one integer min/max with the shifted operand (lanes shifted in
+// from outside the 16-lane row read the identity); lane 15 of each row then
+// holds the row's extreme, and the four rows are combined from scalar reads.
+__device__ inline void wave_minmax(float& mn, float& mx) {
+    int a = f32_key(mn), b = f32_key(mx);
+#define AMPS_DPP_STEP(CTRL)                                                                  \
+    a = min(a, __builtin_amdgcn_update_dpp(0x7FFFFFFF, a, CTRL, 0xF, 0xF, false));          \
+    b = max(b, __builtin_amdgcn_update_dpp((int)0x80000000, b, CTRL, 0xF, 0xF, false))
+    AMPS_DPP_STEP(0x111);  // row_shr:1
+    AMPS_DPP_STEP(0x112);  // row_shr:2
+    AMPS_DPP_STEP(0x114);  // row_shr:4
+    AMPS_DPP_STEP(0x118);  // row_shr:8
+#undef AMPS_DPP_STEP
+    const int a0 = __builtin_amdgcn_readlane(a, 15), a1 = __builtin_amdgcn_readlane(a, 31);
+    const int a2 = __builtin_amdgcn_readlane(a, 47), a3 = __builtin_amdgcn_readlane(a, 63);
+    const int b0 = __builtin_amdgcn_readlane(b, 15), b1 = __builtin_amdgcn_readlane(b, 31);
+    const int b2 = __builtin_amdgcn_readlane(b, 47), b3 = __builtin_amdgcn_readlane(b, 63);
+    mn = key_f32(min(min(a0, a1), min(a2, a3)));
+    mx = key_f32(max(max(b0, b1), max(b2, b3)));
+}
 
 // Min and max over the workgroup; red must hold 2*WG/64 floats.  Contains
 // barriers: every thread must call it.
 __device__ inline void block_minmax(float& mn, float& mx, float* red) {
-    mn = wave_min(mn);
-    mx = wave_max(mx);
+    wave_minmax(mn, mx);
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     __syncthreads();
     if (l == 0) { red[w] = mn; red[4 + w] = mx; }
@@ -247,10 +274,8 @@ __device__ inline void block_minmax(float& mn, float& mx, float* red) {
 // every thread has passed a barrier since it last read red.
 __device__ inline void block_minmax2(float& mn0, float& mx0, float& mn1, float& mx1, float* red) {
     static_assert(WG == 256, "four waves per workgroup");
-    mn0 = wave_min(mn0);
-    mx0 = wave_max(mx0);
-    mn1 = wave_min(mn1);
-    mx1 = wave_max(mx1);
+    wave_minmax(mn0, mx0);
+    wave_minmax(mn1, mx1);
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) *reinterpret_cast<float4*>(red + 4 * w) = make_float4(mn0, mx0, mn1, mx1);
     __syncthreads();
