@@ -345,9 +345,11 @@ int enqueue_monitor(amps_ctx* c, int b) {
     }
     // the monitored field's own precision decides how it is read
     if (c->gm.fg[field0] < 0) a.type = nullptr;
-    RCCHK(launch_fill_u64(c->d_stats + 1, 1, 0ull, c->stream));
-    RCCHK(launch_monitor(a, c->monitor, c->tile, ngroups_eval, field0, stride, c->ngroups, c->d_md, c->d_last, c->stream));
-    RCCHK(launch_classify(c->d_desc, c->d_tile_inst, c->total_tiles, c->ngroups, broadcast, c->d_md, c->thr_eff,
+    // the monitor kernel also clears the classification's per-pass FP64 count
+    RCCHK(launch_monitor(a, c->monitor, c->tile, ngroups_eval, field0, stride, c->ngroups, c->d_md, c->d_last,
+                         c->d_stats + 1, c->stream));
+    RCCHK(launch_classify(c->d_desc, c->d_tile_inst, c->total_tiles, c->ngroups, broadcast, c->ninst == 1 ? 1 : 0,
+                          c->d_md, c->thr_eff,
                           c->d_type_old, c->d_type, c->d_stats, c->stream));
     RCCHK(launch_convert(c->d_desc, c->d_tile_inst, c->buf[b], c->total_cells, c->total_tiles, c->d_type_old,
                          c->d_type, c->gm, CONVERT_CHANGED, c->tile, c->stream));

@@ -203,7 +203,8 @@ __device__ inline double read_decoded(const Buf& b, long long total_cells, int t
 __global__ void monitor_samples_kernel(const InstDesc* inst, const int* tile_inst, Buf b, long long total_cells,
                                        int total_tiles, const int* type, int ngroups_eval, int field0,
                                        int field_stride, int type_groups, int tile, int kind, double* md,
-                                       double* last) {
+                                       double* last, unsigned long long* __restrict__ reset) {
+    if (reset && blockIdx.x == 0 && threadIdx.x == 0) *reset = 0ull;  // counter the classification adds to
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= ngroups_eval * total_tiles) return;
     const int g = w / total_tiles, gtile = w - g * total_tiles;
@@ -263,7 +264,8 @@ __global__ void monitor_last_init_kernel(const InstDesc* inst, const int* tile_i
 // bit.  One workgroup per (group, tile).
 template <int TS>
 __global__ __launch_bounds__(256) void monitor_grad_kernel(StepArgs a, int ngroups_eval, int field0,
-                                                           int field_stride, int type_groups, double* md) {
+                                                           int field_stride, int type_groups, double* md, unsigned long long* __restrict__ reset) {
+    if (reset && blockIdx.x == 0 && threadIdx.x == 0) *reset = 0ull;  // counter the classification adds to
     constexpr int H = 1, LW = TS + 2;
     __shared__ double box[LW * LW];
     __shared__ NbField nbf;
@@ -322,7 +324,8 @@ struct MonWaveLds {
 __global__ __launch_bounds__(64) void monitor_grad_wave16(StepArgs a, const InstDesc* __restrict__ inst,
                                                           const int* __restrict__ tinst,
                                                           const int* __restrict__ types, int ngroups_eval, int field0,
-                                                          int field_stride, int type_groups, double* __restrict__ md) {
+                                                          int field_stride, int type_groups, double* __restrict__ md, unsigned long long* __restrict__ reset) {
+    if (reset && blockIdx.x == 0 && threadIdx.x == 0) *reset = 0ull;  // counter the classification adds to
     constexpr int TS = 16, LWD = MonWaveLds::LWD, C0 = MonWaveLds::C0;
     static_assert(LWD % 2 == 0 && LWD >= C0 + TS + 2, "16-byte reads of columns x0-2 .. x0+5");
     __shared__ __align__(16) MonWaveLds L;
@@ -510,11 +513,13 @@ __global__ __launch_bounds__(64) void monitor_grad_wave16(StepArgs a, const Inst
 // self and the 4 non-periodic von-Neumann neighbours with fmax (CUDA's max);
 // < thr -> 1, >= thr -> 2, unordered keeps the old type.  One thread per
 // (group, tile); with broadcast every group takes group 0's monitor value.
-constexpr int CL_PT = 4;  // (group, tile) items per thread
+constexpr int CL_PT = 1;  // (group, tile) items per thread: one dependent load chain each
 
-__global__ __launch_bounds__(1024) void classify_kernel(const InstDesc* inst, const int* tile_inst, int total_tiles,
-                                                        int ngroups, int broadcast, const double* md, double thr,
-                                                        int* type_old, int* type, unsigned long long* stats) {
+__global__ __launch_bounds__(1024) void classify_kernel(const InstDesc* __restrict__ inst,
+                                                        const int* __restrict__ tile_inst, int total_tiles,
+                                                        int ngroups, int broadcast, int single, const double* md,
+                                                        double thr, int* type_old, int* type,
+                                                        unsigned long long* stats) {
     __shared__ unsigned cnt[16];
     unsigned nhi = 0;
     const int n = ngroups * total_tiles;
@@ -523,7 +528,7 @@ __global__ __launch_bounds__(1024) void classify_kernel(const InstDesc* inst, co
         const int w = (blockIdx.x * CL_PT + k) * blockDim.x + threadIdx.x;
         if (w >= n) continue;
         const int g = w / total_tiles, gtile = w - g * total_tiles;
-        const InstDesc& d = inst[tile_inst[gtile]];
+        const InstDesc& d = inst[single ? 0 : tile_inst[gtile]];  // one instance: no tile -> instance load
         const int lt = gtile - d.tile_base, uy = lt / d.tx, ux = lt - uy * d.tx;
         const double* m0 = md + (size_t)(broadcast ? 0 : g) * total_tiles;
         double m = m0[gtile];
@@ -734,7 +739,8 @@ __device__ inline double mon_rl_d(double v, int l) {
 __global__ __launch_bounds__(64) void monitor_grad_wave32(StepArgs a, const InstDesc* __restrict__ inst,
                                                           const int* __restrict__ tinst,
                                                           const int* __restrict__ types, int field0,
-                                                          double* __restrict__ md) {
+                                                          double* __restrict__ md, unsigned long long* __restrict__ reset) {
+    if (reset && blockIdx.x == 0 && threadIdx.x == 0) *reset = 0ull;  // counter the classification adds to
     constexpr int TS = 32, LWD = 38, C0 = 2;
     __shared__ __align__(16) double box[(TS + 2) * LWD];
     const int lane = threadIdx.x;
@@ -843,14 +849,14 @@ __global__ __launch_bounds__(64) void monitor_grad_wave32(StepArgs a, const Inst
 }
 
 int launch_monitor(const StepArgs& a, int kind, int tile, int ngroups_eval, int field0, int field_stride,
-                   int type_groups, double* md, double* last, hipStream_t s) {
+                   int type_groups, double* md, double* last, unsigned long long* reset, hipStream_t s) {
     const int n = ngroups_eval * a.total_tiles;
     if (kind == AMPS_MON_GRADIENT) {
         int grid = n > 65536 ? 65536 : n;
         if (tile == 16) {
             const int g16 = a.total_tiles > 262144 ? 262144 : a.total_tiles;
             hipLaunchKernelGGL(monitor_grad_wave16, dim3(g16), dim3(64), 0, s, a, a.inst, a.tile_inst, a.type,
-                               ngroups_eval, field0, field_stride, type_groups, md);
+                               ngroups_eval, field0, field_stride, type_groups, md, reset);
         }
         else if (tile == 32 && ngroups_eval == 1) {
             static int per_cu = 0, cus = 0;
@@ -865,15 +871,15 @@ int launch_monitor(const StepArgs& a, int kind, int tile, int ngroups_eval, int 
             }
             const int g32 = per_cu * cus < a.total_tiles ? per_cu * cus : a.total_tiles;
             hipLaunchKernelGGL(monitor_grad_wave32, dim3(g32 > 0 ? g32 : 1), dim3(64), 0, s, a, a.inst, a.tile_inst,
-                               a.type, field0, md);
+                               a.type, field0, md, reset);
         }
-        else if (tile == 32) hipLaunchKernelGGL((monitor_grad_kernel<32>), dim3(grid), dim3(256), 0, s, a, ngroups_eval, field0, field_stride, type_groups, md);
-        else if (tile == 64) hipLaunchKernelGGL((monitor_grad_kernel<64>), dim3(grid), dim3(256), 0, s, a, ngroups_eval, field0, field_stride, type_groups, md);
+        else if (tile == 32) hipLaunchKernelGGL((monitor_grad_kernel<32>), dim3(grid), dim3(256), 0, s, a, ngroups_eval, field0, field_stride, type_groups, md, reset);
+        else if (tile == 64) hipLaunchKernelGGL((monitor_grad_kernel<64>), dim3(grid), dim3(256), 0, s, a, ngroups_eval, field0, field_stride, type_groups, md, reset);
         else return AMPS_E_UNSUPPORTED;
     } else {
         hipLaunchKernelGGL(monitor_samples_kernel, dim3((n + 255) / 256), dim3(256), 0, s, a.inst, a.tile_inst, a.in,
                            a.total_cells, a.total_tiles, a.type, ngroups_eval, field0, field_stride, type_groups, tile,
-                           kind, md, last);
+                           kind, md, last, reset);
     }
     return hip_ok();
 }
@@ -887,11 +893,12 @@ int launch_monitor_last_init(const StepArgs& a, int tile, int ngroups_eval, int 
 }
 
 int launch_classify(const InstDesc* inst, const int* tile_inst, int total_tiles, int ngroups, int broadcast,
-                    const double* md, double thr, int* type_old, int* type, unsigned long long* stats, hipStream_t s) {
+                    int single, const double* md, double thr, int* type_old, int* type, unsigned long long* stats,
+                    hipStream_t s) {
     const int n = ngroups * total_tiles;
     const int per = 1024 * CL_PT;
     hipLaunchKernelGGL(classify_kernel, dim3(n > 0 ? (n + per - 1) / per : 1), dim3(1024), 0, s, inst, tile_inst,
-                       total_tiles, ngroups, broadcast, md, thr, type_old, type, stats);
+                       total_tiles, ngroups, broadcast, single, md, thr, type_old, type, stats);
     return hip_ok();
 }
 

@@ -18,12 +18,16 @@ int launch_init(const InstDesc* inst_dev, int inst, long long ncells, int model,
                 long long total_cells, hipStream_t s);
 int launch_convert(const InstDesc* inst, const int* tile_inst, const Buf& b, long long total_cells, int total_tiles,
                    const int* type_old, const int* type, const GroupMap& gm, int mode, int tile, hipStream_t s);
+// reset (may be null) is set to 0 by the monitor kernel (the counter the
+// classification that follows adds to).
 int launch_monitor(const StepArgs& a, int kind, int tile, int ngroups_eval, int field0, int field_stride,
-                   int type_groups, double* md, double* last, hipStream_t s);
+                   int type_groups, double* md, double* last, unsigned long long* reset, hipStream_t s);
 int launch_monitor_last_init(const StepArgs& a, int tile, int ngroups_eval, int field0, int field_stride,
                              double* last, hipStream_t s);
+// single: one instance (descriptor 0 for every tile).
 int launch_classify(const InstDesc* inst, const int* tile_inst, int total_tiles, int ngroups, int broadcast,
-                    const double* md, double thr, int* type_old, int* type, unsigned long long* stats, hipStream_t s);
+                    int single, const double* md, double thr, int* type_old, int* type, unsigned long long* stats,
+                    hipStream_t s);
 int launch_gather(const InstDesc* inst, const Buf& b, long long total_cells, int total_tiles, const int* type,
                   int group, int instid, int ntiles_inst, int field, double* dst, int tile, hipStream_t s);
 int launch_nonfinite(const double* v, long long n, unsigned long long* cnt, hipStream_t s);
