@@ -142,6 +142,8 @@ def test_driver_usage_errors():
     assert os.access(drv, os.X_OK)
     for args in ([], ["snowflake"], ["nosuchmodel", "purehigh", "1", "x.csv"], ["snowflake", "purehigh", "1"],
                  ["sintering", "purehigh", "1", "x.csv"], ["graingrowth", "puremix"], ["snowflake", "main"],
-                 ["snowflake", "purelow", "--bogus"]):
+                 ["snowflake", "purelow", "--bogus"], ["sintering", "motivation_mix", "x.csv"],
+                 ["snowflake", "motivation_mix"], ["snowflake", "monitor_independent_withmonitor", "1"],
+                 ["sintering", "monitor_independent_withoutmonitor"]):
         r = subprocess.run([drv, *args], capture_output=True, text=True, timeout=30)
         assert r.returncode == 2 and "usage" in r.stderr, (args, r.returncode, r.stderr)

@@ -108,3 +108,116 @@ def test_sintering_main_result_tree(amps, tmp_path):
     assert len(found) == 1 and f"/end2end/double/{sub}" in found[0] and "/para4/" in found[0], found
     ref, _ = api_field(amps, amps.SINTERING, files, cfg.mode, 0)
     assert same_6_digits(read_csv(found[0]), ref)
+
+
+def motivation_types(md, tx, ty, ratio):
+    """The motivation study's tile map (snowflake/motivation_mix.cu:75-103,
+    tools.h:66-106), restated: tiles ranked by monitor value (descending, ties
+    in index order) become FP64 while the value is nonzero, up to
+    int(n - n*ratio); the set then grows breadth first over the 8 periodic
+    neighbours.  The first seed and first grown tile precede the count check."""
+    from collections import deque
+    n = tx * ty
+    high = int(n - n * ratio)
+    order = sorted(range(n), key=lambda i: -md[i])
+    t = [1] * n
+    q, s = deque(), 0
+    for k in order:
+        if md[k] == 0:
+            break
+        t[k] = 2
+        s += 1
+        q.append(k)
+        if s >= high:
+            break
+    steps = {1: (-1, 0), 2: (1, 0), 3: (0, -1), 4: (0, 1), 5: (-1, -1), 6: (1, -1), 7: (1, 1), 8: (-1, 1)}
+    while q:
+        c = q.popleft()
+        for d in range(1, 9):
+            x, y = (c % tx + steps[d][0]) % tx, (c // tx + steps[d][1]) % ty
+            nb = y * tx + x
+            if t[nb] != 2:
+                s += 1
+                t[nb] = 2
+                q.append(nb)
+                if s >= high:
+                    break
+        if s >= high:
+            break
+    return np.array(t, np.int32)
+
+
+def sample_max_diff(cfg, now, last, kind):
+    """get_max_diff1 / get_max_diff2 (snowflake/tools.h:177-211) per tile of
+    an FP64 field on the 4x4 sample lattices; Monitor1 starts at min 1, max 0."""
+    t = cfg.tile
+    tx, ty = cfg.nx // t, cfg.ny // t
+    md = np.zeros(tx * ty)
+    p1 = [0, (t - 1) // 3, 2 * (t - 1) // 3, t - 1]
+    p2 = [(2 * i + 1) * t // 8 for i in range(4)]
+    for j in range(ty):
+        for i in range(tx):
+            if kind == 1:
+                v = now[np.ix_([j * t + o for o in p1], [i * t + o for o in p1])]
+                md[j * tx + i] = max(v.max(), 0.0) - min(v.min(), 1.0)
+            else:
+                rows, cols = [j * t + o for o in p2], [i * t + o for o in p2]
+                md[j * tx + i] = np.abs(last[np.ix_(rows, cols)] - now[np.ix_(rows, cols)]).max()
+    return md
+
+
+@pytest.mark.parametrize("model,files,kind", [
+    ("snowflake", ["total.h", "para2.h"], 2), ("snowflake", ["total.h", "para2.h"], 1),
+    ("graingrowth", ["total.h", "para1.h"], 2)])
+def test_motivation_mix_ranked_map(amps, tmp_path, model, files, kind):
+    """motivation_mix: FP64 steps, the ranked-and-grown tile map, 50 steps on
+    it; the field equals a C-ABI run on the map restated above."""
+    steps, ratio = 20, 0.7
+    mid = amps.SNOWFLAKE if model == "snowflake" else amps.GRAINGROWTH
+    paths = [os.path.join(PARAS, model, f) for f in files]
+    popt = sum((["--paras", f] for f in paths), [])
+    extra = []
+    if kind == 1:
+        mon = tmp_path / "monitor1.h"
+        mon.write_text("#define Monitor1\n")
+        paths.append(str(mon))
+        extra = ["--paras", str(mon)]
+    run_driver(tmp_path, model, "motivation_mix", "m.csv", *popt, *extra, "--steps", str(steps), "--ratio", str(ratio))
+    cfg = amps.Config.default(mid)
+    for f in paths:
+        cfg, _ = amps.Config.from_paras(mid, f, base=cfg)
+    assert cfg.monitor == kind
+    cfg.mode = amps.STATIC_MAP
+    field = 0 if model == "snowflake" else 1
+    tx, ty = cfg.nx // cfg.tile, cfg.ny // cfg.tile
+    with amps.Context(cfg) as ctx:
+        ctx.init()
+        groups = ctx.info().type_groups
+        ctx.set_tile_types(np.full((groups, ty, tx), 2, np.int32))
+        last = None
+        if kind == 2:
+            ctx.step(steps - 9)
+            ctx.sync()
+            last = ctx.get_field(field)
+            ctx.step(9)
+        else:
+            ctx.step(steps)
+        ctx.sync()
+        md = sample_max_diff(cfg, ctx.get_field(field), last, kind)
+        t = motivation_types(md, tx, ty, ratio)
+        assert 0 < (t == 2).sum() < t.size  # the case mixes precisions
+        ctx.set_tile_types(np.tile(t.reshape(1, ty, tx), (groups, 1, 1)))
+        ctx.step(50)
+        ctx.sync()
+        ref = ctx.get_field(field)
+    assert same_6_digits(read_csv(tmp_path / "m.csv"), ref)
+
+
+def test_monitor_independent_timings(amps, tmp_path):
+    files = [os.path.join(PARAS, "snowflake", "total.h"), os.path.join(PARAS, "snowflake", "para2.h")]
+    popt = sum((["--paras", f] for f in files), [])
+    times = {}
+    for drv in ("monitor_independent_withmonitor", "monitor_independent_withoutmonitor"):
+        run_driver(tmp_path, "snowflake", drv, *popt, "--steps", str(STEPS))
+        times[drv] = float(open(tmp_path / "time_tmp.csv").read())
+        assert times[drv] > 0
