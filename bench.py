@@ -106,6 +106,38 @@ def copy_probe():
         hip.hipFree(b)
 
 
+def copy_bandwidth(reps=10):
+    """Achieved device-to-device copy bandwidth on this GPU (GB/s, bytes read
+    plus written) of a 1 GiB buffer with the HIP runtime's copy kernel, as
+    the STREAM-copy reference beside the nominal 8 TB/s peak."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = 1 << 30
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) != 0 or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) != 0:
+        raise RuntimeError("copy bandwidth: hipMalloc failed")
+    try:
+        hip.hipMemset(a, 1, ctypes.c_size_t(n))
+        hip.hipEventCreate(ctypes.byref(e0))
+        hip.hipEventCreate(ctypes.byref(e1))
+        hip.hipMemcpyAsync(b, a, ctypes.c_size_t(n), 3, None)  # hipMemcpyDeviceToDevice, warm-up
+        hip.hipEventRecord(e0, None)
+        for _ in range(reps):
+            hip.hipMemcpyAsync(b, a, ctypes.c_size_t(n), 3, None)
+        hip.hipEventRecord(e1, None)
+        if hip.hipEventSynchronize(e1) != 0:
+            raise RuntimeError("copy bandwidth: copy failed")
+        ms = ctypes.c_float()
+        hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+        return 2.0 * n * reps / (ms.value * 1e-3) / 1e9
+    finally:
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+        hip.hipFree(a)
+        hip.hipFree(b)
+
+
 def traffic_from_profiles(name):
     """HBM bytes per launch of a kernel from the committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), or None."""
@@ -229,6 +261,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-others", action="store_true", help="skip the grain-growth, sintering and ensemble lines")
+    ap.add_argument("--repeats", type=int, default=5,
+                    help="timed runs of each path from the initial state; value is the median")
     ap.add_argument("--traffic-run", action="store_true",
                     help="one adaptive run of warmup + steps only, printing the mixed sweep's algorithmic bytes "
                          "per launch over it (the companion of a rocprofv3 --pmc pass)")
@@ -262,14 +296,22 @@ def main():
                           "alg_bytes_per_launch": (i.fp64_tile_updates * tc * B_FP64
                                                    + i.low_full_tile_updates * tc * B_FP16) / n}))
         return
-    log(f"[bench] adaptive path: {cfg.nx}x{cfg.ny}, warmup {args.warmup}, steps {args.steps}")
+    copy_gbs = copy_bandwidth()
+    log(f"[bench] device copy: {copy_gbs:.0f} GB/s (read + write)")
+    log(f"[bench] adaptive path: {cfg.nx}x{cfg.ny}, warmup {args.warmup}, steps {args.steps}, {args.repeats} runs")
     amp = run_path(amps, amp_cfg, args.warmup, args.steps)
-    log(f"[bench] adaptive: {amp['wall']:.3f} s wall, {amp['ev_ms']:.1f} ms events, hi_frac {amp['info'].hi_frac_mean:.4f}")
+    amp_walls = [amp["wall"]] + [run_path(amps, amp_cfg, args.warmup, args.steps, kernel_pass=False,
+                                          with_fields=False)["wall"] for _ in range(args.repeats - 1)]
+    amp["wall"] = float(np.median(amp_walls))
+    log(f"[bench] adaptive: {amp['wall']:.3f} s wall (median of {len(amp_walls)}), hi_frac {amp['info'].hi_frac_mean:.4f}")
     f64 = None
     if not args.no_fp64:
         log("[bench] FP64 path")
         f64 = run_path(amps, f64_cfg, args.warmup, args.steps)
-        log(f"[bench] fp64: {f64['wall']:.3f} s wall, {f64['ev_ms']:.1f} ms events")
+        f64_walls = [f64["wall"]] + [run_path(amps, f64_cfg, args.warmup, args.steps, kernel_pass=False,
+                                              with_fields=False)["wall"] for _ in range(args.repeats - 1)]
+        f64["wall"] = float(np.median(f64_walls))
+        log(f"[bench] fp64: {f64['wall']:.3f} s wall (median of {len(f64_walls)})")
 
     value = cells * args.steps / amp["wall"]
     out = {
@@ -300,7 +342,10 @@ def main():
                        "traffic": tr["bytes_per_launch"] if tr else None,
                        "kernel": "sf_mixed_kernel<32,false>", "avg_launch_us": mix_s * 1e6,
                        "alg_bytes_per_launch": mix_bytes, "fp64_tiles_per_launch": fp64_tiles,
-                       "low_full_tiles_per_launch": full_tiles}
+                       "low_full_tiles_per_launch": full_tiles, "device_copy_gbs": copy_gbs,
+                       "frac_of_device_copy": achieved / copy_gbs}
+    out["runs"] = {"adaptive_values": [cells * args.steps / w for w in amp_walls],
+                   "note": f"value is the median of {len(amp_walls)} timed runs from the initial state"}
     out["adaptive_step_kernels_us"] = {"sf_mixed_kernel<32,false>": mix_s * 1e6,
                                        "sf_hold_kernel<32,1,false>": hold_s * 1e6,
                                        "note": "per-launch averages over the timed steps, HIP events bound "
@@ -319,7 +364,9 @@ def main():
         out["fp64"] = {"value": f64_value, "ms_per_step": f64["wall"] * 1e3 / args.steps,
                        "speedup_amp_over_fp64": value / f64_value,
                        "roofline": {"achieved": fa, "peak": HBM_PEAK_GBS, "frac": fa / HBM_PEAK_GBS,
-                                    "kernel": "sf_high_kernel<32,false,false>", "avg_launch_us": kf_s * 1e6}}
+                                    "frac_of_device_copy": fa / copy_gbs,
+                                    "kernel": "sf_high_kernel<32,false,false>", "avg_launch_us": kf_s * 1e6},
+                       "runs_values": [cells * args.steps / w for w in f64_walls]}
         out["error_vs_fp64"] = err
     if not args.no_others:
         out["other_workloads"] = other_workloads(amps, log)
