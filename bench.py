@@ -65,7 +65,21 @@ def run_path(amps, cfg, warmup, steps, kernel_pass=True, with_fields=True):
         wall = time.perf_counter() - t0
         info = ctx.info()
         fields = [ctx.get_field(f) for f in range(min(cfg.nfields(), 3))] if with_fields else None
-    res = dict(wall=wall, ev_ms=ev_ms, info=info, fields=fields)
+        grain = None
+        if with_fields and cfg.nfields() > 3:
+            # grain map of an order-parameter model: sum of eta^2 and the index
+            # of the largest eta per cell, accumulated field by field
+            s2 = np.zeros((cfg.ny, cfg.nx))
+            best = np.full((cfg.ny, cfg.nx), -np.inf)
+            gid = np.zeros((cfg.ny, cfg.nx), np.int32)
+            for f in range(cfg.nfields()):
+                v = ctx.get_field(f)
+                s2 += v * v
+                m = v > best
+                best[m] = v[m]
+                gid[m] = f
+            grain = (s2, gid)
+    res = dict(wall=wall, ev_ms=ev_ms, info=info, fields=fields, grain=grain)
     if kernel_pass:
         with amps.Context(cfg) as ctx:
             ctx.init()
@@ -181,6 +195,10 @@ def other_workloads(amps, log):
                  "tile": f["tile"], "wg_per_cu": f["wg_per_cu"]},
         "speedup_amp_over_fp64": f["r"]["wall"] / a["r"]["wall"],
         "rel_l2_eta_fields_0_2": [rel_l2(x, y) for x, y in zip(a["r"]["fields"], f["r"]["fields"])],
+        "grain_map_vs_fp64": {
+            "sum_eta2_rel_l2": rel_l2(a["r"]["grain"][0], f["r"]["grain"][0]),
+            "sum_eta2_linf": float(np.max(np.abs(a["r"]["grain"][0] - f["r"]["grain"][0]))),
+            "grain_id_agreement": float(np.mean(a["r"]["grain"][1] == f["r"]["grain"][1]))},
         "unit": "cell-updates/s"}
     # sintering
     sn, ts = amps.Config.from_paras(amps.SINTERING, os.path.join(P, "sintering", "workload1024.h"))
@@ -212,7 +230,7 @@ def other_workloads(amps, log):
         cfgs.append(c)
     steps = 5000
     cells = sum(c.nx * c.ny for c in cfgs)
-    walls = {}
+    walls, phis = {}, {}
     for name, mode in (("adaptive", amps.ADAPTIVE), ("fp64", amps.PURE_FP64)):
         cs = [c.copy() for c in cfgs]
         for c in cs:
@@ -225,10 +243,13 @@ def other_workloads(amps, log):
             step_chunked(ctx, steps)
             ctx.sync()
             walls[name] = time.perf_counter() - t0
+            phis[name] = [ctx.get_field(0, inst=i) for i in range(len(cs))]
+    rms = [float(np.sqrt(np.mean((x - y) ** 2))) for x, y in zip(phis["adaptive"], phis["fp64"])]
     res["snowflake_ensemble_64"] = {
         "steps": steps, "cells": cells, "instances": 64,
         "adaptive": {"value": cells * steps / walls["adaptive"]}, "fp64": {"value": cells * steps / walls["fp64"]},
-        "speedup_amp_over_fp64": walls["fp64"] / walls["adaptive"], "unit": "cell-updates/s"}
+        "speedup_amp_over_fp64": walls["fp64"] / walls["adaptive"],
+        "phi_l2_rms_vs_fp64": {"median": float(np.median(rms)), "max": max(rms)}, "unit": "cell-updates/s"}
     return res
 
 
