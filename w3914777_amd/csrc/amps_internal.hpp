@@ -147,4 +147,11 @@ __device__ inline int xcd_remap(int b, int n) {
     return b;
 }
 
+// Inverse of xcd_remap: the b with xcd_remap(b, n) == r.
+__device__ inline int xcd_unmap(int r, int n) {
+    int per = n >> 3;
+    if (r < (per << 3)) return (r % per) * 8 + r / per;
+    return r;
+}
+
 }  // namespace amps

@@ -590,7 +590,7 @@ __global__ __launch_bounds__(1024) void worklist_kernel(const int* __restrict__ 
                                                         int total_tiles, int* __restrict__ wl_hi,
                                                         int* __restrict__ wl_lo, int* __restrict__ counts,
                                                         int* __restrict__ same_flags) {
-    __shared__ int ws[2][16];
+    __shared__ int ws[3][16];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int t0 = blockIdx.x * WL_TILES;
     int bh = 0, bl = 0;
@@ -599,20 +599,25 @@ __global__ __launch_bounds__(1024) void worklist_kernel(const int* __restrict__ 
         bh += (v.x == 2) + (v.y == 2) + (v.z == 2) + (v.w == 2);
         bl += (v.x == 1) + (v.y == 1) + (v.z == 1) + (v.w == 1);
     }
+    // FP64 tiles after this block's chunk: the total places the entries
+    int ah = 0;
+    for (int u = t0 + WL_TILES + tid; u < total_tiles; u += WL_TILES) ah += type[u] == 2;
     const int t = t0 + tid;
     const int ty = t < total_tiles ? type[t] : 0;
     const bool hi = ty == 2, lo = ty == 1;
     const unsigned long long mh = __ballot(hi), ml = __ballot(lo);
     bh = __ockl_wfred_add_i32(bh);
     bl = __ockl_wfred_add_i32(bl);
+    ah = __ockl_wfred_add_i32(ah);
     if (lane == 0) {
         ws[0][wave] = bh;
         ws[1][wave] = bl;
+        ws[2][wave] = ah;
     }
     __syncthreads();
-    int oh = 0, ol = 0;
+    int oh = 0, ol = 0, nafter = 0;
 #pragma unroll
-    for (int w = 0; w < 16; ++w) { oh += ws[0][w]; ol += ws[1][w]; }
+    for (int w = 0; w < 16; ++w) { oh += ws[0][w]; ol += ws[1][w]; nafter += ws[2][w]; }
     __syncthreads();
     if (lane == 0) {
         ws[0][wave] = __popcll(mh);
@@ -628,7 +633,10 @@ __global__ __launch_bounds__(1024) void worklist_kernel(const int* __restrict__ 
         tl += y;
     }
     const unsigned long long below = (1ull << lane) - 1ull;
-    if (hi) wl_hi[oh + __popcll(mh & below)] = t;
+    // FP64 entry of rank r (tile order) at the position q with xcd_remap(q, n) == r:
+    // consumers read wl_hi[i] for item i and still give each XCD a contiguous run
+    const int nhi = th + nafter;
+    if (hi) wl_hi[xcd_unmap(oh + __popcll(mh & below), nhi)] = t;
     if (lo) wl_lo[ol + __popcll(ml & below)] = t;
     if (same_flags && t < total_tiles && (!type_old || type_old[t] != ty)) same_flags[t] = 0;
     if (tid == 0 && t0 + WL_TILES >= total_tiles) { counts[0] = th; counts[1] = tl; }

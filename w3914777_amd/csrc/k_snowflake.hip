@@ -958,9 +958,10 @@ __global__ __launch_bounds__(256, 3) void sf_high_kernel(StepArgs a, const InstD
 // Mixed sweep of the adaptive step: the FP64 tiles (FP64 work-list) and the
 // low tiles that failed the hold sweep's bulk-hold test (its 8 segments of
 // this step's parity) are the items of one persistent launch, so the two
-// latency-bound tile chains run side by side.  Items [0, nh) are FP64 tiles,
-// the rest low tiles; the LDS is a union of the two layouts, with a barrier
-// after every item.
+// latency-bound tile chains run side by side.  Items [0, nh) are FP64 tiles
+// (work-list entry i, stored XCD-remapped by the work-list kernel), the rest
+// low tiles; the LDS is a union of the two layouts, with a barrier after
+// every item.
 template <int TS>
 union SfMixLds {
     SfLds<TS, double> h;
@@ -973,19 +974,30 @@ __global__ __launch_bounds__(256, 3) void sf_mixed_kernel(StepArgs a, const Inst
                                                       const int* __restrict__ wlc, const int* __restrict__ fseg,
                                                       const int* __restrict__ fcnt2) {
     __shared__ __align__(16) SfMixLds<TS> U;
+    // Everything the first item needs is read in one round: both parities'
+    // segment counts (no wait on the step counter first) and work-list entry
+    // blockIdx.x (entries sit at XCD-remapped positions, so item i is entry i;
+    // the index and value are clamped in case there is no such FP64 item).
+    const int G = gridDim.x, T = a.total_tiles;
+    const int wfirst = wl[blockIdx.x < (unsigned)T ? (int)blockIdx.x : T - 1];
+    int c0[8], c1[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) { c0[k] = fcnt2[k]; c1[k] = fcnt2[8 + k]; }
+    const int nh = *wlc;
     const long long step = *a.step_ctr + a.step_add;
-    const int* fcnt = fcnt2 + (int)(step & 1) * 8;
+    const bool odd = (step & 1) != 0;
     int cnt[8], nf = 0;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) { cnt[k] = fcnt[k]; nf += cnt[k]; }
-    const int nh = *wlc, n = nh + nf, G = gridDim.x;
+    for (int k = 0; k < 8; ++k) { cnt[k] = odd ? c1[k] : c0[k]; nf += cnt[k]; }
+    const int n = nh + nf;
+    asm volatile("" ::"s"(wfirst));  // the entry's load stays in the counts' round (not sunk into the loop)
     if (blockIdx.x == 0 && threadIdx.x == 0) {  // tile counts of the step (one writer per launch)
         a.stats[2] += (unsigned long long)nh;
         a.stats[3] += (unsigned long long)nf;
     }
     for (int i = blockIdx.x; i < n; i += G) {
         if (i < nh) {
-            const int w = wl[xcd_remap(i, nh)];
+            const int w = i == (int)blockIdx.x ? wfirst : wl[i];
             const InstDesc& d = inst[MULTI ? tinst[w] : 0];
             const int t = nb_type_lane<TS>(a, d, w);
             const int lt = w - d.tile_base;
