@@ -28,6 +28,7 @@ struct SnLds {
     double mu[RW * RW];
     NbField nbf[2];
     double red[4];
+    float redf[4 * WG / 64];  // eta bounds of a low tile (block_minmax2)
 };
 
 __device__ inline int ring_src_sn(int r, int o, int n, int bc) {
@@ -172,15 +173,22 @@ __global__ __launch_bounds__(256) void sn_step_kernel(StepArgs a, const InstDesc
 #pragma unroll
             for (int i = 0; i < PER; ++i) { o1[gcell[i]] = n1[i]; o2[gcell[i]] = n2[i]; }
         } else {
-            // encode both etas of the tile (offset = mid-range, power-of-two scale)
-            for (int k = 0; k < 2; ++k) {
-                double* v = k == 0 ? n1 : n2;
-                double mn = INFINITY, mx = -INFINITY;
+            // encode both etas of the tile (offset = mid-range, power-of-two
+            // scale).  The bounds are the values' min and max rounded outward to
+            // float (a range that covers every value), all four reduced over the
+            // workgroup with one barrier.
+            float lo0 = INFINITY, hi0 = -INFINITY, lo1 = INFINITY, hi1 = -INFINITY;
 #pragma unroll
-                for (int i = 0; i < PER; ++i) { mn = fmin(mn, v[i]); mx = fmax(mx, v[i]); }
-                mx = block_max_d(mx, L.red);
-                __syncthreads();
-                mn = -block_max_d(-mn, L.red);
+            for (int i = 0; i < PER; ++i) {
+                lo0 = fminf(lo0, __double2float_rd(n1[i]));
+                hi0 = fmaxf(hi0, __double2float_ru(n1[i]));
+                lo1 = fminf(lo1, __double2float_rd(n2[i]));
+                hi1 = fmaxf(hi1, __double2float_ru(n2[i]));
+            }
+            block_minmax2(lo0, hi0, lo1, hi1, L.redf);
+            for (int k = 0; k < 2; ++k) {
+                const double* v = k == 0 ? n1 : n2;
+                const double mn = k == 0 ? lo0 : lo1, mx = k == 0 ? hi0 : hi1;
                 double off = 0.5 * (mn + mx), hw = 0.5 * (mx - mn);
                 int ex = hw == 0.0 ? -100 : 0;  // a uniform tile takes the smallest scale
                 if (hw > 0.0) frexp(hw, &ex);
@@ -193,7 +201,6 @@ __global__ __launch_bounds__(256) void sn_step_kernel(StepArgs a, const InstDesc
                     a.out.off[(size_t)(1 + k) * a.total_tiles + w] = off;
                     a.out.scl[(size_t)(1 + k) * a.total_tiles + w] = hw == 0.0 ? 0.0f : ldexpf(1.0f, ex - 14);
                 }
-                __syncthreads();
             }
         }
         __syncthreads();
