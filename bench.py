@@ -120,6 +120,30 @@ def copy_probe():
         hip.hipFree(b)
 
 
+def threshold_sweep(amps, amp_cfg, f64_cfg, args, f64, cells, log, horizons=(2000, 5000, 10000)):
+    """The workload at monitor thresholds 0.01-0.2: rate, speedup over the
+    FP64 run, FP64 tile share and the phi / T RMS error against FP64 at the
+    end of the run and after shorter horizons (before the noisy dendrite's
+    chaotic divergence dominates the difference)."""
+    rms = lambda x, y: float(np.sqrt(np.mean((x - y) ** 2)))
+    refs = {h: run_path(amps, f64_cfg, args.warmup, h, kernel_pass=False)["fields"] for h in horizons}
+    rows = []
+    for thr in (0.01, 0.02, 0.05, 0.1, 0.2):
+        c = amp_cfg.copy()
+        c.threshold = thr
+        r = run_path(amps, c, args.warmup, args.steps, kernel_pass=False)
+        row = {"threshold": thr, "value": cells * args.steps / r["wall"],
+               "speedup_amp_over_fp64": f64["wall"] / r["wall"], "hi_frac_mean": r["info"].hi_frac_mean,
+               "phi_l2_rms": rms(r["fields"][0], f64["fields"][0]), "T_l2_rms": rms(r["fields"][1], f64["fields"][1])}
+        for h in horizons:
+            fs = run_path(amps, c, args.warmup, h, kernel_pass=False)["fields"]
+            row[f"phi_l2_rms_at_{h}"] = rms(fs[0], refs[h][0])
+            row[f"T_l2_rms_at_{h}"] = rms(fs[1], refs[h][1])
+        rows.append(row)
+        log(f"[bench] threshold {thr}: {row['speedup_amp_over_fp64']:.2f}x, hi {row['hi_frac_mean']:.4f}")
+    return rows
+
+
 def copy_bandwidth(reps=10):
     """Achieved device-to-device copy bandwidth on this GPU (GB/s, bytes read
     plus written) of a 1 GiB buffer with the HIP runtime's copy kernel, as
@@ -282,6 +306,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-others", action="store_true", help="skip the grain-growth, sintering and ensemble lines")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the monitor-threshold sweep")
     ap.add_argument("--repeats", type=int, default=5,
                     help="timed runs of each path from the initial state; value is the median")
     ap.add_argument("--traffic-run", action="store_true",
@@ -389,6 +414,8 @@ def main():
                                     "kernel": "sf_high_kernel<32,false,false>", "avg_launch_us": kf_s * 1e6},
                        "runs_values": [cells * args.steps / w for w in f64_walls]}
         out["error_vs_fp64"] = err
+        if not args.no_sweep:
+            out["threshold_sweep"] = threshold_sweep(amps, amp_cfg, f64_cfg, args, f64, cells, log)
     if not args.no_others:
         out["other_workloads"] = other_workloads(amps, log)
     if not args.no_cpu:
