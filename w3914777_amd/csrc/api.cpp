@@ -115,6 +115,7 @@ struct amps_ctx {
     // sweep-kernel timing (events around each step-kernel launch)
     bool ktiming = false;
     std::vector<hipEvent_t> kev;  // quads: start and end of a step's first and second sweep kernel
+    std::vector<unsigned char> kev_pairs;  // per timed step: 1 or 2 sweep kernels (pairs recorded)
     int kev_used = 0;
     long long klaunches = 0;
     double kms = 0.0;                      // all sweep kernels
@@ -278,7 +279,7 @@ int kev_flush(amps_ctx* c) {
     for (int i = 0; i < c->kev_used; ++i) {
         float a = 0.0f, b = 0.0f;
         HIPCHK(hipEventElapsedTime(&a, c->kev[4 * i], c->kev[4 * i + 1]));
-        HIPCHK(hipEventElapsedTime(&b, c->kev[4 * i + 2], c->kev[4 * i + 3]));
+        if (c->kev_pairs[i] == 2) HIPCHK(hipEventElapsedTime(&b, c->kev[4 * i + 2], c->kev[4 * i + 3]));
         c->kms += (double)a + (double)b;
         c->kms_part[0] += a;
         c->kms_part[1] += b;
@@ -304,6 +305,7 @@ int enqueue_step(amps_ctx* c, int from, int step_add, bool capturing = false) {
     a.step_add = step_add;
     if (!c->ktiming || capturing) return launch_step_kernel(c, a);
     if (c->kev_used * 4 + 4 > (int)c->kev.size()) RCCHK(kev_flush(c));
+    c->kev_pairs[c->kev_used] = c->model == AMPS_SNOWFLAKE && has_low(c) ? 2 : 1;
     RCCHK(launch_step_kernel(c, a, &c->kev[4 * c->kev_used]));
     c->kev_used++;
     return AMPS_OK;
@@ -845,6 +847,7 @@ extern "C" int amps_set_kernel_timing(amps_ctx* c, int enable) {
     RCCHK(kev_flush(c));
     if (enable && c->kev.empty()) {
         c->kev.resize(4 * 512);
+        c->kev_pairs.assign(512, 1);
         for (auto& e : c->kev) HIPCHK(hipEventCreate(&e));
     }
     c->ktiming = enable != 0;

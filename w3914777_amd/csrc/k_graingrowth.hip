@@ -695,7 +695,6 @@ static void gg_launch(const StepArgs& a, int grid, hipStream_t s, const hipEvent
         else
             launch_sweep(gg_block_kernel<TS, ADAPT, false>, dim3(grid), dim3(WG), s, ev, a, a.inst, a.tile_inst, a.type);
     }
-    record_empty_pair(s, ev ? ev + 2 : nullptr);
 }
 
 int launch_gg_step(const StepArgs& a, int tile, bool adaptive, int grid, hipStream_t s, const hipEvent_t* ev) {

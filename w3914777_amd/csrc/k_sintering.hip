@@ -213,7 +213,6 @@ static void sn_launch(const StepArgs& a, int grid, hipStream_t s, const hipEvent
         launch_sweep(sn_step_kernel<TS, ADAPT, true>, dim3(grid), dim3(WG), s, ev, a, a.inst, a.tile_inst, a.type);
     else
         launch_sweep(sn_step_kernel<TS, ADAPT, false>, dim3(grid), dim3(WG), s, ev, a, a.inst, a.tile_inst, a.type);
-    record_empty_pair(s, ev ? ev + 2 : nullptr);
 }
 
 int launch_sn_step(const StepArgs& a, int tile, bool adaptive, int grid, hipStream_t s, const hipEvent_t* ev) {

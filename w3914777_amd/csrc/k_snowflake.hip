@@ -1067,7 +1067,6 @@ static void sf_launch(const StepArgs& a, bool adaptive, hipStream_t s, const hip
         constexpr auto kh = sf_high_kernel<TS, false, MULTI>;
         launch_sweep(kh, dim3(resident_grid<kh>(a.nwork, a.grid_cap)), blk, s, ev, a, a.inst, a.tile_inst,
                      (const int*)nullptr, (const int*)nullptr);
-        record_empty_pair(s, ev ? ev + 2 : nullptr);
     }
 }
 

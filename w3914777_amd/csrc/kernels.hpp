@@ -34,10 +34,9 @@ int launch_nonfinite(const double* v, long long n, unsigned long long* cnt, hipS
 int launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t s);
 int launch_advance_step(long long* step_ctr, int n, hipStream_t s);
 
-// ev (may be null): four timing events, the start and end of the step's first
-// sweep kernel and of its second (snowflake adaptive: the bulk-hold sweep,
-// then the mixed sweep); a step with one sweep kernel records the second pair
-// back to back.
+// ev (may be null): timing events bound to the step's sweep kernels: the
+// start and end of the first (snowflake adaptive: the bulk-hold sweep) and,
+// for the snowflake's adaptive step only, of the second (the mixed sweep).
 int launch_sf_step(const StepArgs& a, int tile, bool adaptive, hipStream_t s, const hipEvent_t* ev = nullptr);
 // same_flags[f][tile] = 0 and bit f of zmask[tile] cleared for every field f whose
 // type group changed type on the tile.

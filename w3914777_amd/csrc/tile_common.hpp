@@ -27,13 +27,6 @@ inline void launch_sweep(K k, dim3 grid, dim3 blk, hipStream_t s, const hipEvent
     else hipLaunchKernelGGL(k, grid, blk, 0, s, args...);
 }
 
-// Second timing pair of a step with one sweep kernel: recorded back to back.
-inline void record_empty_pair(hipStream_t s, const hipEvent_t* ev) {
-    if (!ev) return;
-    (void)hipEventRecord(ev[0], s);
-    (void)hipEventRecord(ev[1], s);
-}
-
 // Type of (tile, group); group < 0 means the field is always FP64.
 __device__ inline int tile_type(const StepArgs& a, int group, int gtile) {
     if (a.type == nullptr || group < 0) return 2;
