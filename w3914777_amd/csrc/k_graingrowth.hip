@@ -80,12 +80,16 @@ __device__ inline void gg_halo16(int h, int& by, int& bx) {
     else { by = h - 52; bx = 16; }
 }
 
-// Field data of one field for the lane: own strip (4 FP64 values or 4 fp16
-// words) and up to two halo elements (FP64 bits or an fp16 word).
+// Field data of one field for the lane, loaded by one sequence of loads for
+// both precisions (addresses selected, no branches), so that the loads of the
+// next field stay in flight while the current one is computed: own strip as
+// 4 FP64 values (a: cells 0-1, b: cells 2-3) or, for fp16, the aligned
+// 8-word group holding the 4 own words (loaded twice); each halo element as
+// its FP64 bits or the aligned 4-word group holding its fp16 word.  Grids
+// are multiples of the tile, so every group lies inside its row.
 struct GgRaw {
-    double own[4];
-    uint2 own16;
-    unsigned long long halo[2];
+    uint4 a, b;
+    uint2 halo[2];
 };
 
 template <bool ADAPT>
@@ -94,20 +98,23 @@ __device__ inline void gg_wave_load(GgRaw& r, const StepArgs& a, const InstDesc&
     const double* f64 = a.in.f64 + (size_t)f * a.total_cells + d.cell_base;
     const unsigned short* f16 =
         reinterpret_cast<const unsigned short*>(a.in.f16 + (size_t)f * a.total_cells + d.cell_base);
-    if (ty == 2) {
-        const double2 u = *reinterpret_cast<const double2*>(f64 + cell0);
-        const double2 v = *reinterpret_cast<const double2*>(f64 + cell0 + 2);
-        r.own[0] = u.x; r.own[1] = u.y; r.own[2] = v.x; r.own[3] = v.y;
-    } else {
-        r.own16 = *reinterpret_cast<const uint2*>(f16 + cell0);
-    }
+    const bool own64 = !ADAPT || ty == 2;
+    const uint4* po = own64 ? reinterpret_cast<const uint4*>(f64 + cell0) : reinterpret_cast<const uint4*>(f16 + (cell0 & ~7));
+    r.a = po[0];
+    r.b = po[own64 ? 1 : 0];
 #pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        if (k < nh) {
-            if (!ADAPT || L.ntype[f][hreg[k]] == 2) r.halo[k] = (unsigned long long)__double_as_longlong(f64[hoff[k]]);
-            else r.halo[k] = f16[hoff[k]];
-        }
+    for (int k = 0; k < 2; ++k) {  // a lane with one halo element loads cell hoff[1] = 0 (unused)
+        const bool h64 = !ADAPT || L.ntype[f][hreg[k]] == 2;
+        r.halo[k] = *(h64 ? reinterpret_cast<const uint2*>(f64 + hoff[k])
+                          : reinterpret_cast<const uint2*>(f16 + (hoff[k] & ~3)));
     }
+}
+
+__device__ inline double raw_f64(unsigned lo, unsigned hi) { return __hiloint2double((int)hi, (int)lo); }
+
+// fp16 word of halo element k from its 4-word group (hoff & 3 selects it)
+__device__ inline unsigned raw_halo16(uint2 g, int hoff) {
+    return ((hoff & 2) ? g.y : g.x) >> ((hoff & 1) << 4);
 }
 
 __device__ inline float half_word(unsigned w) { return __half2float(__ushort_as_half((unsigned short)w)); }
@@ -137,7 +144,7 @@ constexpr double GG_FLAT = 1.0e-10;
 // own strip and at most two halo elements (offsets computed once per tile),
 // the next field's loads in flight while the current field is computed.
 template <int TS, bool ADAPT, bool MULTI>
-__global__ __launch_bounds__(64) void gg_wave_kernel(StepArgs a, const InstDesc* __restrict__ inst,
+__global__ __launch_bounds__(64, 4) void gg_wave_kernel(StepArgs a, const InstDesc* __restrict__ inst,
                                                      const int* __restrict__ tinst, const int* __restrict__ types) {
     static_assert(TS == 16, "the one-wave mapping covers 16x16 tiles");
     using Lw = GgWaveLds;
@@ -302,32 +309,33 @@ __global__ __launch_bounds__(64) void gg_wave_kernel(StepArgs a, const InstDesc*
             // stage field f (own strip + halo) into the LDS box
             if (ty == 2) {
                 double* row = L.d + (ly + 1) * LWD + C0D + x0;
-                *reinterpret_cast<double2*>(row) = make_double2(raw.own[0], raw.own[1]);
-                *reinterpret_cast<double2*>(row + 2) = make_double2(raw.own[2], raw.own[3]);
+                *reinterpret_cast<double2*>(row) = make_double2(raw_f64(raw.a.x, raw.a.y), raw_f64(raw.a.z, raw.a.w));
+                *reinterpret_cast<double2*>(row + 2) = make_double2(raw_f64(raw.b.x, raw.b.y), raw_f64(raw.b.z, raw.b.w));
 #pragma unroll
                 for (int k = 0; k < 2; ++k) {
                     if (k < nh) {
                         const int r = hreg[k];
                         L.d[hby[k] * LWD + C0D + hbx[k]] =
                             (!ADAPT || L.ntype[f][r] == 2)
-                                ? __longlong_as_double((long long)raw.halo[k])
-                                : L.noff[f][r] + (double)half_word((unsigned)raw.halo[k]) * (double)L.nscl[f][r];
+                                ? raw_f64(raw.halo[k].x, raw.halo[k].y)
+                                : L.noff[f][r] + (double)half_word(raw_halo16(raw.halo[k], hoff[k])) * (double)L.nscl[f][r];
                     }
                 }
             } else {
                 const double own_off = L.noff[f][4];
                 const float sc = L.nscl[f][4];
                 float* row = L.f + (ly + 1) * LWF + C0F + x0;
+                const uint2 o16 = (x0 & 4) ? make_uint2(raw.a.z, raw.a.w) : make_uint2(raw.a.x, raw.a.y);
                 *reinterpret_cast<float4*>(row) =
-                    make_float4(half_word(raw.own16.x & 0xFFFFu) * sc, half_word(raw.own16.x >> 16) * sc,
-                                half_word(raw.own16.y & 0xFFFFu) * sc, half_word(raw.own16.y >> 16) * sc);
+                    make_float4(half_word(o16.x & 0xFFFFu) * sc, half_word(o16.x >> 16) * sc,
+                                half_word(o16.y & 0xFFFFu) * sc, half_word(o16.y >> 16) * sc);
 #pragma unroll
                 for (int k = 0; k < 2; ++k) {
                     if (k < nh) {
                         const int r = hreg[k];
                         float x;
-                        if (L.ntype[f][r] == 2) x = (float)(__longlong_as_double((long long)raw.halo[k]) - own_off);
-                        else x = (float)(L.noff[f][r] - own_off) + half_word((unsigned)raw.halo[k]) * L.nscl[f][r];
+                        if (L.ntype[f][r] == 2) x = (float)(raw_f64(raw.halo[k].x, raw.halo[k].y) - own_off);
+                        else x = (float)(L.noff[f][r] - own_off) + half_word(raw_halo16(raw.halo[k], hoff[k])) * L.nscl[f][r];
                         L.f[hby[k] * LWF + C0F + hbx[k]] = x;
                     }
                 }
@@ -335,7 +343,8 @@ __global__ __launch_bounds__(64) void gg_wave_kernel(StepArgs a, const InstDesc*
             {
                 // next updated field's loads in flight while this one is computed
                 const int fn = f + 1 < a.q ? (int)__builtin_ctzll((live >> (f + 1) << (f + 1)) | (1ull << 63)) : a.q;
-                if (fn < a.q) gg_wave_load<ADAPT>(raw, a, d, fn, ADAPT ? L.ntype[fn][4] : 2, cell0, hoff, hreg, nh, L);
+                const int fl = fn < a.q ? fn : f;  // past the last field: loads of this field again (unused)
+                gg_wave_load<ADAPT>(raw, a, d, fl, ADAPT ? L.ntype[fl][4] : 2, cell0, hoff, hreg, nh, L);
             }
             __syncthreads();
             if (ty == 2) {
