@@ -415,29 +415,30 @@ __global__ __launch_bounds__(64) void monitor_grad_wave16(StepArgs a, const Inst
         }
         __syncthreads();
         const double r = d.p[0] / d.p[1];  // dx / dy (slots 0 and 1 of every model)
-        // raw loads of field g: FP64 strip or fp16 words, halo FP64 bits or fp16 word
-        double ownd[4];
-        uint2 ownh = make_uint2(0u, 0u);
-        unsigned long long hraw[2] = {0ull, 0ull};
+        // raw loads of field g, one sequence for both precisions (addresses
+        // selected, no branches) so that they stay in flight while the previous
+        // field is evaluated: own strip as 4 FP64 values (ra: cells 0-1, rb:
+        // cells 2-3) or the aligned 8-word fp16 group holding the 4 own words;
+        // halo elements as FP64 bits or the aligned 4-word fp16 group (grids
+        // are multiples of the tile, so every group lies inside its row; a lane
+        // with one halo element loads cell hoff[1] = 0, unused)
+        uint4 ra = make_uint4(0u, 0u, 0u, 0u), rb = ra;
+        uint2 rh[2] = {make_uint2(0u, 0u), make_uint2(0u, 0u)};
         auto load = [&](int g) {
             const int field = field0 + g * field_stride;
             const double* f64 = a.in.f64 + (size_t)field * a.total_cells + d.cell_base;
             const unsigned short* f16 =
                 reinterpret_cast<const unsigned short*>(a.in.f16 + (size_t)field * a.total_cells + d.cell_base);
-            if (uni || L.ntype[g][4] == 2) {
-                const double2 u = *reinterpret_cast<const double2*>(f64 + cell0);
-                const double2 v = *reinterpret_cast<const double2*>(f64 + cell0 + 2);
-                ownd[0] = u.x; ownd[1] = u.y; ownd[2] = v.x; ownd[3] = v.y;
-            } else {
-                ownh = *reinterpret_cast<const uint2*>(f16 + cell0);
-            }
+            const bool own64 = uni || L.ntype[g][4] == 2;
+            const uint4* po = own64 ? reinterpret_cast<const uint4*>(f64 + cell0)
+                                    : reinterpret_cast<const uint4*>(f16 + (cell0 & ~7));
+            ra = po[0];
+            rb = po[own64 ? 1 : 0];
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
-                if (k < nh) {
-                    if (uni || L.ntype[g][hreg[k]] == 2)
-                        hraw[k] = (unsigned long long)__double_as_longlong(f64[hoff[k]]);
-                    else hraw[k] = f16[hoff[k]];
-                }
+                const bool h64 = uni || L.ntype[g][hreg[k]] == 2;
+                rh[k] = *(h64 ? reinterpret_cast<const uint2*>(f64 + hoff[k])
+                              : reinterpret_cast<const uint2*>(f16 + (hoff[k] & ~3)));
             }
         };
         const unsigned long long live = ~z5;
@@ -447,9 +448,13 @@ __global__ __launch_bounds__(64) void monitor_grad_wave16(StepArgs a, const Inst
             if ((z5 >> g) & 1ull) continue;
             double own[4];
             if (uni || L.ntype[g][4] == 2) {
-                own[0] = ownd[0]; own[1] = ownd[1]; own[2] = ownd[2]; own[3] = ownd[3];
+                own[0] = raw_f64(ra.x, ra.y);
+                own[1] = raw_f64(ra.z, ra.w);
+                own[2] = raw_f64(rb.x, rb.y);
+                own[3] = raw_f64(rb.z, rb.w);
             } else {
                 const double o = L.noff[g][4], sc = (double)L.nscl[g][4];
+                const uint2 ownh = raw_words8(ra, x0);
                 own[0] = o + (double)__half2float(__ushort_as_half((unsigned short)(ownh.x & 0xFFFFu))) * sc;
                 own[1] = o + (double)__half2float(__ushort_as_half((unsigned short)(ownh.x >> 16))) * sc;
                 own[2] = o + (double)__half2float(__ushort_as_half((unsigned short)(ownh.y & 0xFFFFu))) * sc;
@@ -460,10 +465,10 @@ __global__ __launch_bounds__(64) void monitor_grad_wave16(StepArgs a, const Inst
             for (int k = 0; k < 2; ++k) {
                 if (k < nh) {
                     const int rg = hreg[k];
-                    if (uni || L.ntype[g][rg] == 2) hv[k] = __longlong_as_double((long long)hraw[k]);
+                    if (uni || L.ntype[g][rg] == 2) hv[k] = raw_f64(rh[k].x, rh[k].y);
                     else
-                        hv[k] = L.noff[g][rg] +
-                                (double)__half2float(__ushort_as_half((unsigned short)hraw[k])) * (double)L.nscl[g][rg];
+                        hv[k] = L.noff[g][rg] + (double)__half2float(__ushort_as_half((unsigned short)raw_word4(rh[k], hoff[k]))) *
+                                                    (double)L.nscl[g][rg];
                 }
             }
             double* row = L.box + (ly + 1) * LWD + C0 + x0;
@@ -476,7 +481,7 @@ __global__ __launch_bounds__(64) void monitor_grad_wave16(StepArgs a, const Inst
                 // next evaluated field in flight during this one
                 const int gn = g + 1 < ngroups_eval ? (int)__builtin_ctzll((live >> (g + 1) << (g + 1)) | (1ull << 63))
                                                     : ngroups_eval;
-                if (gn < ngroups_eval) load(gn);
+                load(gn < ngroups_eval ? gn : g);  // past the last field: this field again (unused)
             }
             __syncthreads();
             double m = 0.0;

@@ -110,13 +110,6 @@ __device__ inline void gg_wave_load(GgRaw& r, const StepArgs& a, const InstDesc&
     }
 }
 
-__device__ inline double raw_f64(unsigned lo, unsigned hi) { return __hiloint2double((int)hi, (int)lo); }
-
-// fp16 word of halo element k from its 4-word group (hoff & 3 selects it)
-__device__ inline unsigned raw_halo16(uint2 g, int hoff) {
-    return ((hoff & 2) ? g.y : g.x) >> ((hoff & 1) << 4);
-}
-
 __device__ inline float half_word(unsigned w) { return __half2float(__ushort_as_half((unsigned short)w)); }
 
 // Held field-tiles (adaptive step): a low field-tile whose tile code and the
@@ -318,14 +311,14 @@ __global__ __launch_bounds__(64, 4) void gg_wave_kernel(StepArgs a, const InstDe
                         L.d[hby[k] * LWD + C0D + hbx[k]] =
                             (!ADAPT || L.ntype[f][r] == 2)
                                 ? raw_f64(raw.halo[k].x, raw.halo[k].y)
-                                : L.noff[f][r] + (double)half_word(raw_halo16(raw.halo[k], hoff[k])) * (double)L.nscl[f][r];
+                                : L.noff[f][r] + (double)half_word(raw_word4(raw.halo[k], hoff[k])) * (double)L.nscl[f][r];
                     }
                 }
             } else {
                 const double own_off = L.noff[f][4];
                 const float sc = L.nscl[f][4];
                 float* row = L.f + (ly + 1) * LWF + C0F + x0;
-                const uint2 o16 = (x0 & 4) ? make_uint2(raw.a.z, raw.a.w) : make_uint2(raw.a.x, raw.a.y);
+                const uint2 o16 = raw_words8(raw.a, x0);
                 *reinterpret_cast<float4*>(row) =
                     make_float4(half_word(o16.x & 0xFFFFu) * sc, half_word(o16.x >> 16) * sc,
                                 half_word(o16.y & 0xFFFFu) * sc, half_word(o16.y >> 16) * sc);
@@ -335,7 +328,7 @@ __global__ __launch_bounds__(64, 4) void gg_wave_kernel(StepArgs a, const InstDe
                         const int r = hreg[k];
                         float x;
                         if (L.ntype[f][r] == 2) x = (float)(raw_f64(raw.halo[k].x, raw.halo[k].y) - own_off);
-                        else x = (float)(L.noff[f][r] - own_off) + half_word(raw_halo16(raw.halo[k], hoff[k])) * L.nscl[f][r];
+                        else x = (float)(L.noff[f][r] - own_off) + half_word(raw_word4(raw.halo[k], hoff[k])) * L.nscl[f][r];
                         L.f[hby[k] * LWF + C0F + hbx[k]] = x;
                     }
                 }

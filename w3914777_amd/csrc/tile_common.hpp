@@ -227,6 +227,22 @@ __device__ inline int f32_key(float v) {
 }
 __device__ inline float key_f32(int k) { return __int_as_float(k ^ ((k >> 31) & 0x7FFFFFFF)); }
 
+// Raw field data loaded as plain words (one load sequence for FP64 and fp16
+// tiles): an FP64 value from its two 32-bit halves as stored (low first); the
+// fp16 word at position p & 3 of an aligned 4-word group; the 4 words at
+// position p & 4 of an aligned 8-word group.
+__device__ inline double raw_f64(unsigned lo, unsigned hi) { return __hiloint2double((int)hi, (int)lo); }
+
+__device__ inline unsigned raw_word4(uint2 g, int p) {
+    const unsigned long long q = ((unsigned long long)g.y << 32) | g.x;
+    return (unsigned)(q >> ((p & 3) << 4)) & 0xFFFFu;
+}
+
+__device__ inline uint2 raw_words8(uint4 g, int p) {
+    const bool up = (p & 4) != 0;
+    return make_uint2(up ? g.z : g.x, up ? g.w : g.y);
+}
+
 // Min and max of non-NaN floats over a full wave (all 64 lanes active),
 // returned in every lane.  Integer keys let each step of the row_shr DPP
 // ladder be one integer min/max with the shifted operand (lanes shifted in
