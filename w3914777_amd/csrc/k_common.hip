@@ -339,7 +339,7 @@ __global__ __launch_bounds__(64) void monitor_grad_wave16(StepArgs a, const Inst
         const int ox = txi * TS, oy = tyi * TS;
         const bool interior = ox >= 1 && oy >= 1 && ox + TS + 1 <= d.nx && oy + TS + 1 <= d.ny;
         const int cell0 = (oy + ly) * d.nx + ox + x0;
-        int hoff[2] = {0, 0}, hreg[2] = {4, 4}, hidx[2] = {0, 0};
+        int hoff[2] = {cell0, cell0}, hreg[2] = {4, 4}, hidx[2] = {0, 0};  // no element: own cell (unused)
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             if (k < nh) {
@@ -421,7 +421,7 @@ __global__ __launch_bounds__(64) void monitor_grad_wave16(StepArgs a, const Inst
         // cells 2-3) or the aligned 8-word fp16 group holding the 4 own words;
         // halo elements as FP64 bits or the aligned 4-word fp16 group (grids
         // are multiples of the tile, so every group lies inside its row; a lane
-        // with one halo element loads cell hoff[1] = 0, unused)
+        // with one halo element loads its own cell again, unused)
         uint4 ra = make_uint4(0u, 0u, 0u, 0u), rb = ra;
         uint2 rh[2] = {make_uint2(0u, 0u), make_uint2(0u, 0u)};
         auto load = [&](int g) {

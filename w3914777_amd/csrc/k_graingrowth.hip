@@ -86,7 +86,9 @@ __device__ inline void gg_halo16(int h, int& by, int& bx) {
 // 4 FP64 values (a: cells 0-1, b: cells 2-3) or, for fp16, the aligned
 // 8-word group holding the 4 own words (loaded twice); each halo element as
 // its FP64 bits or the aligned 4-word group holding its fp16 word.  Grids
-// are multiples of the tile, so every group lies inside its row.
+// are multiples of the tile, so every group lies inside its row.  A lane
+// with one halo element reads its own cell for the second (a cache hit; one
+// shared address for every such lane would load a single memory channel).
 struct GgRaw {
     uint4 a, b;
     uint2 halo[2];
@@ -103,7 +105,7 @@ __device__ inline void gg_wave_load(GgRaw& r, const StepArgs& a, const InstDesc&
     r.a = po[0];
     r.b = po[own64 ? 1 : 0];
 #pragma unroll
-    for (int k = 0; k < 2; ++k) {  // a lane with one halo element loads cell hoff[1] = 0 (unused)
+    for (int k = 0; k < 2; ++k) {  // a lane with one halo element loads its own cell0 again (unused)
         const bool h64 = !ADAPT || L.ntype[f][hreg[k]] == 2;
         r.halo[k] = *(h64 ? reinterpret_cast<const uint2*>(f64 + hoff[k])
                           : reinterpret_cast<const uint2*>(f16 + (hoff[k] & ~3)));
@@ -158,7 +160,7 @@ __global__ __launch_bounds__(64, 4) void gg_wave_kernel(StepArgs a, const InstDe
         const bool moore = d.stencil == AMPS_STENCIL_REF;
         const int cell0 = (oy + ly) * d.nx + ox + x0;
         // halo elements of the lane: cell offsets, regions, LDS rows/columns
-        int hoff[2] = {0, 0}, hreg[2] = {4, 4}, hby[2] = {0, 0}, hbx[2] = {0, 0};
+        int hoff[2] = {cell0, cell0}, hreg[2] = {4, 4}, hby[2] = {0, 0}, hbx[2] = {0, 0};
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             if (k < nh) {

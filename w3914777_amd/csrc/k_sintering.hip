@@ -75,8 +75,7 @@ __global__ __launch_bounds__(256) void sn_step_kernel(StepArgs a, const InstDesc
             // its current storage, the 1-cell halo ring from both storages
             // (selected by the neighbour table loaded alongside)
             constexpr int NC = BoxMap<TS, 2>::N, NE = BoxMap<TS, 1>::N;
-            nbfield_fill(&L.nbf[0], a, a.in, nb, 1, 0);
-            nbfield_fill(&L.nbf[1], a, a.in, nb, 2, 0);
+            nbfield_fill2(&L.nbf[0], &L.nbf[1], a, a.in, nb, 1, 2, 0);
             const __half* h1 = a.in.f16 + a.total_cells + d.cell_base;
             const __half* h2 = a.in.f16 + 2 * a.total_cells + d.cell_base;
             double vc[NC], v1[NE], v2[NE];

@@ -226,6 +226,31 @@ __device__ inline void sf_halo_map(SfHalo<TS>& hm, const InstDesc& d, int ox, in
     }
 }
 
+// Halo elements of the frame as raw words, loaded by one sequence for both
+// storages (addresses selected, no branches): a branch per storage makes
+// the compiler wait for each group of loads before issuing the next.  An
+// element of an FP64 tile is its 8 bytes, one of a low tile the aligned
+// 4-word fp16 group holding it (grids are multiples of the tile, so the
+// group lies inside its row).  The loads go straight into the registers
+// they are decoded from, so the branch around a thread's missing element
+// needs no wait at its join.
+template <int TS, bool ADAPT>
+__device__ inline void sf_halo_load(uint2* hp, uint2* ht, const StepArgs& a, const InstDesc& d, const SfHalo<TS>& hm,
+                                    unsigned hmask) {
+    const double* fphi = a.in.f64 + d.cell_base;
+    const double* fT = a.in.f64 + a.total_cells + d.cell_base;
+    const unsigned short* hphi = reinterpret_cast<const unsigned short*>(a.in.f16 + d.cell_base);
+    const unsigned short* hT = reinterpret_cast<const unsigned short*>(a.in.f16 + a.total_cells + d.cell_base);
+#pragma unroll
+    for (int k = 0; k < SfGeo<TS>::NHT; ++k) {
+        const int off = hm.off[k];
+        if (off < 0) continue;
+        const bool h64 = !ADAPT || ((hmask >> hm.reg[k]) & 1u);
+        hp[k] = *(h64 ? reinterpret_cast<const uint2*>(fphi + off) : reinterpret_cast<const uint2*>(hphi + (off & ~3)));
+        ht[k] = *(h64 ? reinterpret_cast<const uint2*>(fT + off) : reinterpret_cast<const uint2*>(hT + (off & ~3)));
+    }
+}
+
 template <int TS, bool ADAPT>
 __device__ void sf_tile_f64(SfLds<TS, double>& L, const StepArgs& a, const InstDesc& d, int tile_x, int tile_y,
                             unsigned hmask, long long step) {
@@ -252,27 +277,14 @@ __device__ void sf_tile_f64(SfLds<TS, double>& L, const StepArgs& a, const InstD
     }
     SfHalo<TS> hm;
     sf_halo_map<TS>(hm, d, ox, oy);
-    double hp[G::NHT], ht[G::NHT];
-    unsigned short hph[G::NHT], hth[G::NHT];
-    const unsigned short* hphi = reinterpret_cast<const unsigned short*>(a.in.f16 + d.cell_base);
-    const unsigned short* hT = reinterpret_cast<const unsigned short*>(a.in.f16 + a.total_cells + d.cell_base);
-#pragma unroll
-    for (int k = 0; k < G::NHT; ++k) {
-        if (hm.off[k] >= 0) {
-            if (!ADAPT || ((hmask >> hm.reg[k]) & 1u)) {
-                hp[k] = fphi[hm.off[k]];
-                ht[k] = fT[hm.off[k]];
-            } else {
-                hph[k] = hphi[hm.off[k]];
-                hth[k] = hT[hm.off[k]];
-            }
-        }
-    }
+    // halo elements as raw words, one load sequence for both storages (see
+    // sf_halo_load): FP64 bits or the aligned 4-word fp16 group
+    uint2 hp[G::NHT], ht[G::NHT];
+    sf_halo_load<TS, ADAPT>(hp, ht, a, d, hm, hmask);
     if (ADAPT) {
         Nb nb;
         nb_setup(nb, d, tile_x, tile_y, TS);
-        nbfield_fill(&L.nbf[0], a, a.in, nb, 0, 0);
-        nbfield_fill(&L.nbf[1], a, a.in, nb, 1, 0);
+        nbfield_fill2(&L.nbf[0], &L.nbf[1], a, a.in, nb, 0, 1, 0);
         __syncthreads();
     }
 #pragma unroll
@@ -285,11 +297,14 @@ __device__ void sf_tile_f64(SfLds<TS, double>& L, const StepArgs& a, const InstD
         if (hm.off[k] < 0) continue;
         const int r = hm.reg[k];
         if (!ADAPT || ((hmask >> r) & 1u)) {
-            L.phi[hm.box[k]] = hp[k];
-            L.T[hm.box[k]] = ht[k];
+            L.phi[hm.box[k]] = raw_f64(hp[k].x, hp[k].y);
+            L.T[hm.box[k]] = raw_f64(ht[k].x, ht[k].y);
         } else {
-            L.phi[hm.box[k]] = L.nbf[0].off[r] + (double)__half2float(__ushort_as_half(hph[k])) * (double)L.nbf[0].scl[r];
-            L.T[hm.box[k]] = L.nbf[1].off[r] + (double)__half2float(__ushort_as_half(hth[k])) * (double)L.nbf[1].scl[r];
+            const int off = hm.off[k];
+            L.phi[hm.box[k]] =
+                L.nbf[0].off[r] + (double)__half2float(__ushort_as_half((unsigned short)raw_word4(hp[k], off))) * (double)L.nbf[0].scl[r];
+            L.T[hm.box[k]] =
+                L.nbf[1].off[r] + (double)__half2float(__ushort_as_half((unsigned short)raw_word4(ht[k], off))) * (double)L.nbf[1].scl[r];
         }
     }
     __syncthreads();
@@ -361,7 +376,7 @@ template <int TS>
 struct LowFetch {
     using G = SfGeo<TS>;
     unsigned short op[G::R], ot[G::R];              // own fp16 words (phi, T)
-    double hp[G::NHT], ht[G::NHT];                  // halo values: FP64, or the fp16 word's bits
+    uint2 hp[G::NHT], ht[G::NHT];                   // halo elements: FP64 bits or the fp16 word's 4-word group
     double offp, offT;
     float sclp, sclT;
 };
@@ -375,8 +390,6 @@ __device__ inline void low_fetch(LowFetch<TS>& f, const StepArgs& a, const InstD
     const int ox = tile_x * TS, oy = tile_y * TS;
     const int ly = threadIdx.x / G::TPR, x0 = (threadIdx.x % G::TPR) * R;
     const int gcell0 = (oy + ly) * d.nx + ox + x0;
-    const double* fphi = a.in.f64 + d.cell_base;
-    const double* fT = a.in.f64 + a.total_cells + d.cell_base;
     const unsigned short* hphi = reinterpret_cast<const unsigned short*>(a.in.f16 + d.cell_base);
     const unsigned short* hT = reinterpret_cast<const unsigned short*>(a.in.f16 + a.total_cells + d.cell_base);
     if (R == 4) {
@@ -390,18 +403,7 @@ __device__ inline void low_fetch(LowFetch<TS>& f, const StepArgs& a, const InstD
     }
     SfHalo<TS> hm;
     sf_halo_map<TS>(hm, d, ox, oy);
-#pragma unroll
-    for (int k = 0; k < G::NHT; ++k) {
-        if (hm.off[k] >= 0) {
-            if ((hmask >> hm.reg[k]) & 1u) {
-                f.hp[k] = fphi[hm.off[k]];
-                f.ht[k] = fT[hm.off[k]];
-            } else {
-                f.hp[k] = __longlong_as_double((long long)hphi[hm.off[k]]);
-                f.ht[k] = __longlong_as_double((long long)hT[hm.off[k]]);
-            }
-        }
-    }
+    sf_halo_load<TS, true>(f.hp, f.ht, a, d, hm, hmask);
     f.offp = a.in.off[w];
     f.offT = a.in.off[a.total_tiles + w];
     f.sclp = a.in.scl[w];
@@ -497,11 +499,11 @@ __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const 
         const int r = hm.reg[k];
         float vp;
         if ((hmask >> r) & 1u) {
-            vp = (float)(f.hp[k] - offp);
-            L.T[hm.box[k]] = (float)(f.ht[k] - offT);
+            vp = (float)(raw_f64(f.hp[k].x, f.hp[k].y) - offp);
+            L.T[hm.box[k]] = (float)(raw_f64(f.ht[k].x, f.ht[k].y) - offT);
         } else {
-            const unsigned short wp = (unsigned short)__double_as_longlong(f.hp[k]);
-            const unsigned short wt = (unsigned short)__double_as_longlong(f.ht[k]);
+            const unsigned short wp = (unsigned short)raw_word4(f.hp[k], hm.off[k]);
+            const unsigned short wt = (unsigned short)raw_word4(f.ht[k], hm.off[k]);
             vp = (float)(L.nbo[slot][0][r] - offp) + __half2float(__ushort_as_half(wp)) * L.nbs[slot][0][r];
             L.T[hm.box[k]] =
                 (float)(L.nbo[slot][1][r] - offT) + __half2float(__ushort_as_half(wt)) * L.nbs[slot][1][r];

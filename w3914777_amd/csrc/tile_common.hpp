@@ -88,6 +88,27 @@ __device__ inline void nbfield_fill(NbField* s, const StepArgs& a, const Buf& b,
     }
 }
 
+// Two fields of one type group: every load of both tables is issued before
+// the first LDS store, so the two tables cost one round of loads.
+__device__ inline void nbfield_fill2(NbField* s0, NbField* s1, const StepArgs& a, const Buf& b, const Nb& nb,
+                                     int field0, int field1, int group) {
+    int t = threadIdx.x;
+    if (t < 9) {
+        int g = nb_tile(nb, t);
+        int ty = tile_type(a, group, g);
+        double off0 = b.off ? b.off[(size_t)field0 * a.total_tiles + g] : 0.0;
+        double off1 = b.off ? b.off[(size_t)field1 * a.total_tiles + g] : 0.0;
+        float scl0 = b.scl ? b.scl[(size_t)field0 * a.total_tiles + g] : 0.0f;
+        float scl1 = b.scl ? b.scl[(size_t)field1 * a.total_tiles + g] : 0.0f;
+        s0->type[t] = ty;
+        s0->off[t] = off0;
+        s0->scl[t] = scl0;
+        s1->type[t] = ty;
+        s1->off[t] = off1;
+        s1->scl[t] = scl1;
+    }
+}
+
 // ------------------------------------------------------- batched box load
 // Offsets (inside one instance's field array) of the box elements a thread
 // stages: element e = threadIdx.x + k*WG.  Computed once per tile and shared
