@@ -737,9 +737,16 @@ __device__ inline void mon_meta_fetch(MonMeta& m, const StepArgs& a, const InstD
     Nb nb;
     nb_setup(nb, d, txi, tyi, 32);
     const int g = nb_tile(nb, lane < 9 ? lane : 8);  // every lane loads: no branch around the loads
-    m.type = types ? types[g] : 2;
-    m.off = types ? a.in.off[(size_t)field * a.total_tiles + g] : 0.0;
-    m.scl = types ? a.in.scl[(size_t)field * a.total_tiles + g] : 0.0f;
+    // without a type map the loads read the first descriptor (a valid
+    // address) and the values are replaced: no branch around the loads
+    const bool has = types != nullptr;
+    const size_t idx = (size_t)field * a.total_tiles + g;
+    const int tv = *(has ? types + g : reinterpret_cast<const int*>(a.inst));
+    const double ov = *(has ? a.in.off + idx : reinterpret_cast<const double*>(a.inst));
+    const float sv = *(has ? a.in.scl + idx : reinterpret_cast<const float*>(a.inst));
+    m.type = has ? tv : 2;
+    m.off = has ? ov : 0.0;
+    m.scl = has ? sv : 0.0f;
 }
 
 __device__ inline double mon_rl_d(double v, int l) {
@@ -761,12 +768,11 @@ __global__ __launch_bounds__(64) void monitor_grad_wave32(StepArgs a, const Inst
     const int n = a.total_tiles, NG = gridDim.x;
     int w = blockIdx.x;
     if (w >= n) return;
-    MonMeta mc;
-    mon_meta_fetch(mc, a, inst[tinst[w]], types, field0, w);
-    for (; w < n; w += NG) {
-        const int wn = w + NG < n ? w + NG : w;
-        MonMeta mn;
-        mon_meta_fetch(mn, a, inst[tinst[wn]], types, field0, wn);
+    // the codes are fetched one tile ahead, alternating between two sets
+    // (ma, mb) so that no copy of a set in flight is needed: a copy waits for
+    // its loads, and a wait on the vector-memory counter also waits for
+    // every load issued after them
+    auto tile = [&](const MonMeta& mc, int w) {
         const InstDesc& d = inst[tinst[w]];
         // uniform low tile between uniform equal edge neighbours: value 0
         const double own_off = mon_rl_d(mc.off, 4);
@@ -774,8 +780,7 @@ __global__ __launch_bounds__(64) void monitor_grad_wave32(StepArgs a, const Inst
         const unsigned long long flat = __ballot(edge && mc.type == 1 && mc.scl == 0.0f && mc.off == own_off);
         if ((flat & 0xBAull) == 0xBAull) {
             if (lane == 0) md[w] = 0.0;
-            mc = mn;
-            continue;
+            return;
         }
         const int lt = w - d.tile_base, tyi = lt / d.tx, txi = lt - tyi * d.tx;
         const int ox = txi * TS, oy = tyi * TS;
@@ -785,22 +790,22 @@ __global__ __launch_bounds__(64) void monitor_grad_wave32(StepArgs a, const Inst
             reinterpret_cast<const unsigned short*>(a.in.f16 + (size_t)field0 * a.total_cells + d.cell_base);
         const int cell0 = (oy + ly) * d.nx + ox + x0;
         const int town = __builtin_amdgcn_readlane(mc.type, 4);
-        // loads: own strip (FP64: 8 x 16 bytes, fp16: 2 x 16 bytes) and two halo elements
-        double own[16];
-        uint4 oh0 = make_uint4(0u, 0u, 0u, 0u), oh1 = oh0;
-        if (town == 2) {
+        // loads: own strip as 8 x 16 bytes, FP64 values or (fp16) the 2 x 16
+        // bytes of words read 4 times over, one load sequence for both storages
+        // (a branch per storage made the compiler wait for the next tile's
+        // codes before the FP64 loads), and two halo elements
+        uint4 ro[8];
+        {
+            const uint4* po = town == 2 ? reinterpret_cast<const uint4*>(f64 + cell0)
+                                        : reinterpret_cast<const uint4*>(f16 + cell0);
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const double2 v = *reinterpret_cast<const double2*>(f64 + cell0 + 2 * q);
-                own[2 * q] = v.x;
-                own[2 * q + 1] = v.y;
-            }
-        } else {
-            oh0 = reinterpret_cast<const uint4*>(f16 + cell0)[0];
-            oh1 = reinterpret_cast<const uint4*>(f16 + cell0)[1];
+            for (int q = 0; q < 8; ++q) ro[q] = po[town == 2 ? q : (q & 1)];
         }
-        unsigned long long hraw[2];
-        int hbox[2], hreg[2];
+        double own[16];
+        // halo elements as raw words by one load sequence for both storages:
+        // FP64 bits or the aligned 4-word fp16 group holding the word
+        uint2 hraw[2];
+        int hbox[2], hreg[2], hoff[2];
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             const int e = lane + 64 * k, band = e >> 5, i = e & 31;
@@ -814,13 +819,20 @@ __global__ __launch_bounds__(64) void monitor_grad_wave32(StepArgs a, const Inst
             hreg[k] = band == 0 ? 1 : (band == 1 ? 7 : (band == 2 ? 3 : 5));
             hbox[k] = (by + 1) * LWD + C0 + bx;
             const int off = gy * d.nx + gx;
+            hoff[k] = off;
             const int rt = __shfl(mc.type, hreg[k], 64);  // this element's region type
-            if (rt == 2) hraw[k] = (unsigned long long)__double_as_longlong(f64[off]);
-            else hraw[k] = f16[off];
+            hraw[k] = *(rt == 2 ? reinterpret_cast<const uint2*>(f64 + off)
+                                : reinterpret_cast<const uint2*>(f16 + (off & ~3)));
         }
-        if (town != 2) {
+        if (town == 2) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                own[2 * q] = raw_f64(ro[q].x, ro[q].y);
+                own[2 * q + 1] = raw_f64(ro[q].z, ro[q].w);
+            }
+        } else {
             const double o = own_off, sc = (double)__shfl(mc.scl, 4, 64);
-            const unsigned wd[8] = {oh0.x, oh0.y, oh0.z, oh0.w, oh1.x, oh1.y, oh1.z, oh1.w};
+            const unsigned wd[8] = {ro[0].x, ro[0].y, ro[0].z, ro[0].w, ro[1].x, ro[1].y, ro[1].z, ro[1].w};
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
                 const unsigned v = wd[j >> 1];
@@ -837,8 +849,9 @@ __global__ __launch_bounds__(64) void monitor_grad_wave32(StepArgs a, const Inst
             const int rt = __shfl(mc.type, rg, 64);
             const double ro = __shfl(mc.off, rg, 64);
             const float rs = __shfl(mc.scl, rg, 64);
-            box[hbox[k]] = rt == 2 ? __longlong_as_double((long long)hraw[k])
-                                   : ro + (double)__half2float(__ushort_as_half((unsigned short)hraw[k])) * (double)rs;
+            box[hbox[k]] = rt == 2 ? raw_f64(hraw[k].x, hraw[k].y)
+                                   : ro + (double)__half2float(__ushort_as_half((unsigned short)raw_word4(hraw[k], hoff[k]))) *
+                                              (double)rs;
         }
         __syncthreads();  // one wave: orders the box writes before the reads
         const double r = d.p[0] / d.p[1];  // dx / dy (slots 0 and 1 of every model)
@@ -857,7 +870,20 @@ __global__ __launch_bounds__(64) void monitor_grad_wave32(StepArgs a, const Inst
         m = wave_max_d(m);
         if (lane == 0) md[w] = m;
         __syncthreads();  // the box is rewritten by the next tile
-        mc = mn;
+    };
+    MonMeta ma, mb;
+    mon_meta_fetch(ma, a, inst[tinst[w]], types, field0, w);
+    for (;;) {
+        int wn = w + NG < n ? w + NG : w;
+        mon_meta_fetch(mb, a, inst[tinst[wn]], types, field0, wn);
+        tile(ma, w);
+        w += NG;
+        if (w >= n) break;
+        wn = w + NG < n ? w + NG : w;
+        mon_meta_fetch(ma, a, inst[tinst[wn]], types, field0, wn);
+        tile(mb, w);
+        w += NG;
+        if (w >= n) break;
     }
 }
 
