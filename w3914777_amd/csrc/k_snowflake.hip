@@ -836,7 +836,10 @@ __device__ void sf_hold_tile(float* Tb, const StepArgs& a, const InstDesc& d, in
 // all of them in one round: lane 9 j + k holds neighbour k of the group's
 // tile j.  Holding, flatness and the same_flags of the group are decided
 // together; only the tiles that need work touch their fields.
-constexpr int HOLD_GT = 7;  // 7 tiles x 9 neighbours <= 64 lanes
+// 4 tiles x 9 neighbours (36 lanes): more, shorter groups run in parallel
+// (4096^2 snowflake: 7.2 us a launch with 7 tiles a group, 6.7 with 3 or 4,
+// 7.4 with 2)
+constexpr int HOLD_GT = 4;
 
 template <int TS, int H, bool MULTI>
 __global__ __launch_bounds__(64) void sf_hold_kernel(StepArgs a, const InstDesc* __restrict__ inst,
@@ -853,7 +856,7 @@ __global__ __launch_bounds__(64) void sf_hold_kernel(StepArgs a, const InstDesc*
     const int ngr = (n + HOLD_GT - 1) / HOLD_GT;
     const int lane = threadIdx.x & 63, tj = lane / 9, tk = lane - 9 * tj;
     for (int g = blockIdx.x; g < ngr; g += gridDim.x) {
-        // tile of this lane's group slot (lane 63 repeats slot 6's tile)
+        // tile of this lane's group slot (lanes past the last slot repeat it)
         const int jj = tj < HOLD_GT ? tj : HOLD_GT - 1;
         const int pj = g + jj * ngr;
         const int pc = pj < n ? pj : g;  // an absent slot repeats slot 0 (loaded, never used)
