@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define AMPS_ABI_VERSION 1
+#define AMPS_ABI_VERSION 2
 
 /* ---- status codes (replace CHECK_ERROR/exit(EXIT_FAILURE),
  *      snowflake/paras/total.h:48-65) ---- */
@@ -91,6 +91,23 @@ enum {
 };
 #define AMPS_NPARAMS 32
 
+/* amps_config.flags bits.
+ * NO_SKIP   : low-precision tiles run the full update every step.  Without it
+ *             the engine skips work it bounds as negligible: snowflake tiles
+ *             whose phi is bulk (within 1e-8 of 0 or 1 on the tile and its 8
+ *             neighbours) advance only T, and T is held where it is flat within
+ *             1e-12; the low-tile noise term is skipped where |phi(1-phi)| <=
+ *             1e-6; grain-growth fields within 1e-10 of 0 on 3x3 tiles are set
+ *             to exactly 0, and grain interiors are held.  The reference updates
+ *             every type-1 tile every step (snowflake/function.h:376-604,
+ *             graingrowth/function.h:33-277): NO_SKIP is that schedule.
+ * CH_CONSERVATIVE : sintering Cahn-Hilliard update in divergence form,
+ *             c' = c + dt div(M grad mu) with face mobilities
+ *             M_{i+1/2} = (M_i + M_{i+1})/2, in FP64 (mass-conserving up to
+ *             the [0,1] clamp).  Without it, the reference's c' = c + dt M lap mu
+ *             (sintering/function.h:95-101). */
+enum { AMPS_FLAG_NO_SKIP = 1, AMPS_FLAG_CH_CONSERVATIVE = 2 };
+
 /* Field indices: snowflake 0 = phi, 1 = T; grain growth 0..q-1 = eta_i;
  * sintering 0 = c, 1 = eta_1, 2 = eta_2. */
 
@@ -109,7 +126,11 @@ typedef struct {
     int32_t init;           /* amps_init_kind                                      */
     int32_t wg_per_cu;      /* sweep workgroups per CU (sets tiles per workgroup);
                                0 = as many as fit (occupancy maximum)              */
-    int32_t reserved;       /* must be 0                                           */
+    int32_t flags;          /* AMPS_FLAG_* bits; 0 = defaults                      */
+    int32_t dilate;         /* ADAPTIVE: the FP64 set also takes every tile within
+                               this Chebyshev distance (in tiles) of a tile whose
+                               monitor value reaches the threshold; 0 = only the
+                               reference's von-Neumann neighbour rule             */
     double threshold;       /* monitor threshold                                   */
     uint64_t seed;          /* counter-based RNG seed (noise, random init)         */
     double params[AMPS_NPARAMS];

@@ -75,6 +75,10 @@ def test_tiling_arguments_are_validated(amps):
         with pytest.raises(amps.AmpsError):
             amps.Context(c)
     c.wg_per_cu = 0
-    c.reserved = 1
+    c.flags = 1 << 7  # no such flag
+    with pytest.raises(amps.AmpsError):
+        amps.Context(c)
+    c.flags = 0
+    c.dilate = -1
     with pytest.raises(amps.AmpsError):
         amps.Context(c)

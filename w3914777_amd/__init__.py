@@ -25,6 +25,7 @@ SNOWFLAKE, GRAINGROWTH, SINTERING = 1, 2, 3
 PURE_FP64, PURE_HALF2, ADAPTIVE, STATIC_MAP = 0, 1, 2, 3
 MON_NONE, MON_SPATIAL, MON_TEMPORAL, MON_GRADIENT = 0, 1, 2, 3
 SWEEP_ALL, SWEEP_FP64, SWEEP_LOW = 0, 1, 2
+FLAG_NO_SKIP, FLAG_CH_CONSERVATIVE = 1, 2
 STENCIL_REF, STENCIL_LAP5 = 0, 1
 BC_PERIODIC, BC_NEUMANN = 0, 1
 INIT_REF, INIT_CONFIG = 0, 1
@@ -55,7 +56,7 @@ class Config(ctypes.Structure):
                 ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("q", ctypes.c_int32), ("tile", ctypes.c_int32),
                 ("monitor_every", ctypes.c_int32), ("monitor_field", ctypes.c_int32),
                 ("stencil", ctypes.c_int32), ("bc", ctypes.c_int32), ("init", ctypes.c_int32),
-                ("wg_per_cu", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("wg_per_cu", ctypes.c_int32), ("flags", ctypes.c_int32), ("dilate", ctypes.c_int32),
                 ("threshold", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("params", ctypes.c_double * NPARAMS)]
 

@@ -130,6 +130,7 @@ struct StepArgs {
     int ninst;                  // instances in the context (1: descriptor is loop-invariant)
     int grid_cap;               // sweep workgroups at most (0: no cap)
     int t_halo;                 // snowflake: halo depth of the T Laplacian (2: reference stencil in some instance)
+    int flags;                  // AMPS_FLAG_* of the run (NO_SKIP: no hold, flush or skip on low tiles)
     Buf in, out;
     double* aux_in;             // model auxiliary per-cell array (grain growth: sum of squares)
     double* aux_out;

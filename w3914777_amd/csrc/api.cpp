@@ -60,6 +60,8 @@ struct amps_ctx {
     int model = 0, mode = 0, monitor = 0, tile = 16, every = 10, mfield = 0;
     int nfields = 0, ngroups = 1, q = 0, ninst = 0;
     int grid_cap = 0;                      // sweep workgroups at most (wg_per_cu x CUs; 0: no cap)
+    int flags = 0;                         // AMPS_FLAG_* of the run
+    int dilate = 0;                        // FP64-set dilation radius in tiles (ADAPTIVE)
     double thr_eff = 0;
     std::vector<amps_config> cfg;
     std::vector<InstDesc> desc;
@@ -154,8 +156,11 @@ int validate(const amps_config& c) {
     if (c.bc != AMPS_BC_PERIODIC && c.bc != AMPS_BC_NEUMANN) return fail(AMPS_E_ARG, "bc out of range");
     if (c.init != AMPS_INIT_REF && c.init != AMPS_INIT_CONFIG) return fail(AMPS_E_ARG, "init out of range");
     if (!finite_params(c)) return fail(AMPS_E_ARG, "non-finite parameter");
-    if (c.wg_per_cu < 0 || c.wg_per_cu > 64 || c.reserved != 0)
-        return fail(AMPS_E_ARG, "wg_per_cu must be in [0, 64] and reserved 0");
+    if (c.wg_per_cu < 0 || c.wg_per_cu > 64) return fail(AMPS_E_ARG, "wg_per_cu must be in [0, 64]");
+    if (c.flags & ~(AMPS_FLAG_NO_SKIP | AMPS_FLAG_CH_CONSERVATIVE)) return fail(AMPS_E_ARG, "unknown flags bit");
+    if ((c.flags & AMPS_FLAG_CH_CONSERVATIVE) && c.model != AMPS_SINTERING)
+        return fail(AMPS_E_ARG, "CH_CONSERVATIVE applies to the sintering model only");
+    if (c.dilate < 0 || c.dilate > 16) return fail(AMPS_E_ARG, "dilate must be in [0, 16]");
     if (c.mode == AMPS_ADAPTIVE) {
         if (c.monitor < 1 || c.monitor > 3) return fail(AMPS_E_ARG, "ADAPTIVE needs monitor 1, 2 or 3");
         if (c.monitor_every < 1) return fail(AMPS_E_ARG, "monitor_every must be >= 1");
@@ -233,6 +238,12 @@ void derive(InstDesc& d, const amps_config& c) {
 
 bool has_low(const amps_ctx* c) { return c->mode != AMPS_PURE_FP64; }
 
+// The snowflake's low-precision step runs two sweep kernels (bulk-hold sweep,
+// then the mixed sweep) unless NO_SKIP runs every tile in the mixed sweep.
+bool two_sweeps(const amps_ctx* c) {
+    return c->model == AMPS_SNOWFLAKE && has_low(c) && !(c->flags & AMPS_FLAG_NO_SKIP);
+}
+
 StepArgs step_args(amps_ctx* c, int from) {
     StepArgs a{};
     a.inst = c->d_desc;
@@ -266,6 +277,7 @@ StepArgs step_args(amps_ctx* c, int from) {
     for (const amps_config& k : c->cfg)
         if (k.stencil == AMPS_STENCIL_REF) a.t_halo = 2;
     a.grid_cap = c->grid_cap;
+    a.flags = c->flags;
     a.in = c->buf[from];
     a.out = c->buf[from ^ 1];
     a.aux_in = c->d_aux[from];
@@ -305,7 +317,7 @@ int enqueue_step(amps_ctx* c, int from, int step_add, bool capturing = false) {
     a.step_add = step_add;
     if (!c->ktiming || capturing) return launch_step_kernel(c, a);
     if (c->kev_used * 4 + 4 > (int)c->kev.size()) RCCHK(kev_flush(c));
-    c->kev_pairs[c->kev_used] = c->model == AMPS_SNOWFLAKE && has_low(c) ? 2 : 1;
+    c->kev_pairs[c->kev_used] = two_sweeps(c) ? 2 : 1;
     RCCHK(launch_step_kernel(c, a, &c->kev[4 * c->kev_used]));
     c->kev_used++;
     return AMPS_OK;
@@ -351,7 +363,7 @@ int enqueue_monitor(amps_ctx* c, int b) {
     RCCHK(launch_monitor(a, c->monitor, c->tile, ngroups_eval, field0, stride, c->ngroups, c->d_md, c->d_last,
                          c->d_stats + 1, c->stream));
     RCCHK(launch_classify(c->d_desc, c->d_tile_inst, c->total_tiles, c->ngroups, broadcast, c->ninst == 1 ? 1 : 0,
-                          c->d_md, c->thr_eff,
+                          c->d_md, c->thr_eff, c->dilate,
                           c->d_type_old, c->d_type, c->d_stats, c->stream));
     RCCHK(launch_convert(c->d_desc, c->d_tile_inst, c->buf[b], c->total_cells, c->total_tiles, c->d_type_old,
                          c->d_type, c->gm, CONVERT_CHANGED, c->tile, c->stream));
@@ -509,7 +521,7 @@ int amps_create_batch(const amps_config* cfgs, int n, amps_ctx** out) {
         const amps_config& b = cfgs[0];
         if (a.model != b.model || a.mode != b.mode || a.tile != b.tile || a.monitor != b.monitor ||
             a.monitor_every != b.monitor_every || a.monitor_field != b.monitor_field || a.threshold != b.threshold ||
-            a.wg_per_cu != b.wg_per_cu ||
+            a.wg_per_cu != b.wg_per_cu || a.flags != b.flags || a.dilate != b.dilate ||
             (a.model == AMPS_GRAINGROWTH && a.q != b.q))
             return fail(AMPS_E_ARG, "batched instances must share model, mode, tile, monitor settings and q");
     }
@@ -522,6 +534,8 @@ int amps_create_batch(const amps_config* cfgs, int n, amps_ctx** out) {
     c->tile = cfgs[0].tile;
     c->every = cfgs[0].monitor_every > 0 ? cfgs[0].monitor_every : 10;
     c->mfield = cfgs[0].monitor_field;
+    c->flags = cfgs[0].flags;
+    c->dilate = cfgs[0].dilate;
     c->q = c->model == AMPS_GRAINGROWTH ? cfgs[0].q : (c->model == AMPS_SINTERING ? 2 : 0);
     c->nfields = nfields_of(cfgs[0]);
     c->ngroups = c->model == AMPS_GRAINGROWTH ? c->q : 1;
@@ -867,7 +881,7 @@ extern "C" int amps_get_sweep_time(amps_ctx* c, int which, int64_t* launches, do
     RCCHK(kev_flush(c));
     *launches = c->klaunches;
     // the snowflake's adaptive step times the low (bulk-hold) sweep first
-    const bool low_first = c->model == AMPS_SNOWFLAKE && has_low(c);
+    const bool low_first = two_sweeps(c);
     const int part = (which == AMPS_SWEEP_FP64) == low_first ? 1 : 0;
     *total_ms = which == AMPS_SWEEP_ALL ? c->kms : c->kms_part[part];
     return AMPS_OK;
@@ -1029,10 +1043,27 @@ namespace {
 std::mutex g_tune_mu;
 std::map<std::string, amps_tuning> g_tune_cache;
 
+// Every field of every config except the two the tuner varies (tile,
+// wg_per_cu), byte for byte, plus the step count: a tiling is reused only
+// for the same run.
+template <typename T>
+void key_add(std::string& k, const T& v) {
+    k.append(reinterpret_cast<const char*>(&v), sizeof(v));
+}
+
 std::string tune_key(const amps_config* c, int n, int steps) {
-    std::string k = std::to_string(c[0].model) + "/" + std::to_string(c[0].mode) + "/" + std::to_string(c[0].monitor) +
-                    "/" + std::to_string(c[0].q) + "/" + std::to_string(steps) + "/" + std::to_string(n);
-    for (int i = 0; i < n; ++i) k += ":" + std::to_string(c[i].nx) + "x" + std::to_string(c[i].ny);
+    std::string k;
+    key_add(k, steps);
+    key_add(k, n);
+    for (int i = 0; i < n; ++i) {
+        const amps_config& x = c[i];
+        for (int32_t v : {x.model, x.mode, x.monitor, x.nx, x.ny, x.q, x.monitor_every, x.monitor_field, x.stencil,
+                          x.bc, x.init, x.flags, x.dilate})
+            key_add(k, v);
+        key_add(k, x.threshold);
+        key_add(k, x.seed);
+        for (double p : x.params) key_add(k, p);
+    }
     return k;
 }
 }  // namespace

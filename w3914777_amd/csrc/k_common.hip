@@ -523,7 +523,7 @@ constexpr int CL_PT = 1;  // (group, tile) items per thread: one dependent load 
 __global__ __launch_bounds__(1024) void classify_kernel(const InstDesc* __restrict__ inst,
                                                         const int* __restrict__ tile_inst, int total_tiles,
                                                         int ngroups, int broadcast, int single, const double* md,
-                                                        double thr, int* type_old, int* type,
+                                                        double thr, int dilate, int* type_old, int* type,
                                                         unsigned long long* stats) {
     __shared__ unsigned cnt[16];
     unsigned nhi = 0;
@@ -541,6 +541,14 @@ __global__ __launch_bounds__(1024) void classify_kernel(const InstDesc* __restri
         if (uy > 0) m = fmax(m, m0[gtile - d.tx]);
         if (ux < d.tx - 1) m = fmax(m, m0[gtile + 1]);
         if (ux > 0) m = fmax(m, m0[gtile - 1]);
+        // dilation of the FP64 set: the maximum over the tiles within
+        // Chebyshev distance `dilate` (non-periodic, inside the instance)
+        if (dilate > 0) {
+            const int y0 = max(uy - dilate, 0), y1 = min(uy + dilate, d.ty - 1);
+            const int x0 = max(ux - dilate, 0), x1 = min(ux + dilate, d.tx - 1);
+            for (int yy = y0; yy <= y1; ++yy)
+                for (int xx = x0; xx <= x1; ++xx) m = fmax(m, m0[d.tile_base + yy * d.tx + xx]);
+        }
         int t = type[w];
         type_old[w] = t;
         if (m < thr) t = 1;
@@ -932,12 +940,12 @@ int launch_monitor_last_init(const StepArgs& a, int tile, int ngroups_eval, int 
 }
 
 int launch_classify(const InstDesc* inst, const int* tile_inst, int total_tiles, int ngroups, int broadcast,
-                    int single, const double* md, double thr, int* type_old, int* type, unsigned long long* stats,
-                    hipStream_t s) {
+                    int single, const double* md, double thr, int dilate, int* type_old, int* type,
+                    unsigned long long* stats, hipStream_t s) {
     const int n = ngroups * total_tiles;
     const int per = 1024 * CL_PT;
     hipLaunchKernelGGL(classify_kernel, dim3(n > 0 ? (n + per - 1) / per : 1), dim3(1024), 0, s, inst, tile_inst,
-                       total_tiles, ngroups, broadcast, single, md, thr, type_old, type, stats);
+                       total_tiles, ngroups, broadcast, single, md, thr, dilate, type_old, type, stats);
     return hip_ok();
 }
 

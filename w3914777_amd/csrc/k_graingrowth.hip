@@ -236,7 +236,9 @@ __global__ __launch_bounds__(64, 4) void gg_wave_kernel(StepArgs a, const InstDe
         if (ADAPT) {
             bool h0 = lane < a.q, h1 = lane < a.q;
             if (lane < a.q && ((zall >> lane) & 1ull)) {
-                h1 = false;  // zero on all 9 tiles (codes not loaded)
+                h1 = false;  // zero on all 9 tiles (codes not loaded): the update keeps it exactly 0
+            } else if (lane < a.q && (a.flags & AMPS_FLAG_NO_SKIP)) {
+                h0 = h1 = false;  // no flush to 0 and no interior hold
             } else if (lane < a.q) {
 #pragma unroll
                 for (int k = 0; k < 9; ++k) {

@@ -464,7 +464,9 @@ __device__ inline void nb_code_store(SfLowLds<TS>& L, int slot, const NbCode& c)
 // done; the next tile's writes of the flags, the reduction slots and P/Q/E2
 // all come after its own barrier A, which every thread reaches only after
 // its last read of this tile's values.
-template <int TS>
+// NOSKIP: no bulk hold and no noise skip (every term of the update is
+// evaluated, as the reference's type-1 branch does).
+template <int TS, bool NOSKIP>
 __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const StepArgs& a, const InstDesc& d,
                             int gtile, const LowFetch<TS>& f, unsigned hmask, long long step) {
     using G = SfGeo<TS>;
@@ -520,7 +522,7 @@ __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const 
     unsigned fl = L.hflag[0];
 #pragma unroll
     for (int i = 1; i < WG / 64; ++i) fl &= L.hflag[i];
-    const bool hold = fl != 0u;
+    const bool hold = !NOSKIP && fl != 0u;
     nb_code_store<TS>(L, slot ^ 1, nxt);
     __half* dphi = a.out.f16 + d.cell_base;
     __half* dT = a.out.f16 + a.total_cells + d.cell_base;
@@ -562,7 +564,7 @@ __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const 
                          d.stencil == AMPS_STENCIL_REF, d.kf[SFK_INV_DXDY], d.kf[SFK_INV_2DX], d.kf[SFK_INV_2DY],
                          d.kf[SFK_DT_OVER_TAU], d.kf[SFK_DT], d.kf[SFK_KAPPA], d.kf[SFK_ALPHA_OVER_PI],
                          d.kf[SFK_GAMMA], (float)(d.k[SFK_TEQ] - offT), (float)offp, d.kf[SFK_NOISE],
-                         SF_NOISE_SKIP_LOW, noise_key(d.seed, (uint64_t)step), gcell0);
+                         NOSKIP ? 0.0f : SF_NOISE_SKIP_LOW, noise_key(d.seed, (uint64_t)step), gcell0);
     float mnp = INFINITY, mxp = -INFINITY, mnt = INFINITY, mxt = -INFINITY;
 #pragma unroll
     for (int i = 0; i < R; ++i) {
@@ -973,7 +975,11 @@ union SfMixLds {
     SfLowLds<TS> l;
 };
 
-template <int TS, bool MULTI>
+//
+// NOSKIP (no hold sweep runs): the low items are the whole low work-list
+// (fseg = the list, fcnt2 = its count), taken XCD-remapped like the FP64
+// sweep's tiles, and every one runs the full low update.
+template <int TS, bool MULTI, bool NOSKIP>
 __global__ __launch_bounds__(256, 3) void sf_mixed_kernel(StepArgs a, const InstDesc* __restrict__ inst,
                                                       const int* __restrict__ tinst, const int* __restrict__ wl,
                                                       const int* __restrict__ wlc, const int* __restrict__ fseg,
@@ -986,14 +992,22 @@ __global__ __launch_bounds__(256, 3) void sf_mixed_kernel(StepArgs a, const Inst
     const int G = gridDim.x, T = a.total_tiles;
     const int wfirst = wl[blockIdx.x < (unsigned)T ? (int)blockIdx.x : T - 1];
     int c0[8], c1[8];
+    if (NOSKIP) {
+        c0[0] = c1[0] = *fcnt2;
+    } else {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) { c0[k] = fcnt2[k]; c1[k] = fcnt2[8 + k]; }
+        for (int k = 0; k < 8; ++k) { c0[k] = fcnt2[k]; c1[k] = fcnt2[8 + k]; }
+    }
     const int nh = *wlc;
     const long long step = *a.step_ctr + a.step_add;
     const bool odd = (step & 1) != 0;
     int cnt[8], nf = 0;
+    if (NOSKIP) {
+        nf = c0[0];
+    } else {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) { cnt[k] = odd ? c1[k] : c0[k]; nf += cnt[k]; }
+        for (int k = 0; k < 8; ++k) { cnt[k] = odd ? c1[k] : c0[k]; nf += cnt[k]; }
+    }
     const int n = nh + nf;
     asm volatile("" ::"s"(wfirst));  // the entry's load stays in the counts' round (not sunk into the loop)
     if (blockIdx.x == 0 && threadIdx.x == 0) {  // tile counts of the step (one writer per launch)
@@ -1009,12 +1023,17 @@ __global__ __launch_bounds__(256, 3) void sf_mixed_kernel(StepArgs a, const Inst
             const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
             sf_tile_f64<TS, true>(U.h, a, d, tile_x, tile_y, hmask_of(t), step);
         } else {
-            // entry j of the concatenated segments
-            int j = i - nh, seg = 0;
+            int w;
+            if (NOSKIP) {
+                w = fseg[xcd_remap(i - nh, nf)];
+            } else {
+                // entry j of the concatenated segments
+                int j = i - nh, seg = 0;
 #pragma unroll
-            for (int k = 0; k < 7; ++k)
-                if (seg == k && j >= cnt[k]) { j -= cnt[k]; seg = k + 1; }
-            const int w = fseg[seg * a.full_cap + j];
+                for (int k = 0; k < 7; ++k)
+                    if (seg == k && j >= cnt[k]) { j -= cnt[k]; seg = k + 1; }
+                w = fseg[seg * a.full_cap + j];
+            }
             const InstDesc& d = inst[MULTI ? tinst[w] : 0];
             const int t = nb_type_lane<TS>(a, d, w);
             NbCode c0;
@@ -1024,9 +1043,74 @@ __global__ __launch_bounds__(256, 3) void sf_mixed_kernel(StepArgs a, const Inst
             low_fetch<TS>(f, a, d, w, hmask);
             nb_code_store<TS>(U.l, 0, c0);
             __syncthreads();  // neighbour table visible
-            sf_tile_low<TS>(U.l, 0, c0, a, d, w, f, hmask, step);
+            sf_tile_low<TS, NOSKIP>(U.l, 0, c0, a, d, w, f, hmask, step);
         }
         __syncthreads();  // the LDS union is rewritten by the next item
+    }
+}
+
+// Full low sweep of the NO_SKIP step: every tile of the low work-list runs
+// the full low update (sf_tile_low<TS, true>).  Persistent workgroups walk
+// the list at XCD-remapped positions with a two-deep software pipeline:
+// while tile k computes, the fp16 words, halo elements and codes of tile
+// k+1 are already in flight (their addresses need tile k+1's neighbour
+// types, which were loaded during tile k-1), so a tile's memory latency
+// overlaps the previous tile's arithmetic instead of following it.
+template <int TS, bool MULTI>
+__global__ __launch_bounds__(256, 4) void sf_lowfull_kernel(StepArgs a, const InstDesc* __restrict__ inst,
+                                                            const int* __restrict__ tinst, const int* __restrict__ wl,
+                                                            const int* __restrict__ wlc) {
+    __shared__ __align__(16) SfLowLds<TS> L;
+    const long long step = *a.step_ctr + a.step_add;
+    const int n = *wlc, G = gridDim.x;
+    int b = blockIdx.x;
+    if (b >= n) return;
+    // tile k = w (fields in f, codes in slot), tile k+1 = w1 (types t1)
+    int w = wl[xcd_remap(b, n)];
+    const bool more1 = b + G < n;
+    int w1 = more1 ? wl[xcd_remap(b + G, n)] : w;
+    unsigned hm;
+    LowFetch<TS> f;
+    {
+        const InstDesc& d = inst[MULTI ? tinst[w] : 0];
+        const int t = nb_type_lane<TS>(a, d, w);
+        NbCode c;
+        nb_code_fetch<TS>(c, a, d, w);
+        hm = hmask_of(t);
+        low_fetch<TS>(f, a, d, w, hm);
+        nb_code_store<TS>(L, 0, c);
+    }
+    int t1 = nb_type_lane<TS>(a, inst[MULTI ? tinst[w1] : 0], w1);
+    __syncthreads();  // slot 0 visible
+    int slot = 0;
+    for (; b < n; b += G) {
+        const bool more = b + G < n;
+        const int b2 = b + 2 * G;
+        const int w2 = b2 < n ? wl[xcd_remap(b2, n)] : w1;
+        // tile k+1: its fields and codes (issued before tile k computes),
+        // tile k+2: its neighbour types
+        const InstDesc& d1 = inst[MULTI ? tinst[w1] : 0];
+        const unsigned hm1 = hmask_of(t1);
+        LowFetch<TS> f1;
+        NbCode c1;
+        if (more) {
+            low_fetch<TS>(f1, a, d1, w1, hm1);
+            nb_code_fetch<TS>(c1, a, d1, w1);
+        } else {
+            f1 = f;
+            c1 = NbCode{0.0, 0.0, 0.0f, 0.0f};
+        }
+        const int t2 = b2 < n ? nb_type_lane<TS>(a, inst[MULTI ? tinst[w2] : 0], w2) : 2;
+        const InstDesc& d = inst[MULTI ? tinst[w] : 0];
+        // stores c1 into slot ^ 1 after its first barrier, which its later
+        // barriers make visible (no barrier at the end: see sf_tile_low)
+        sf_tile_low<TS, true>(L, slot, c1, a, d, w, f, hm, step);
+        slot ^= 1;
+        w = w1;
+        w1 = w2;
+        hm = hm1;
+        t1 = t2;
+        f = f1;
     }
 }
 
@@ -1053,10 +1137,18 @@ static int resident_grid(int nwork, int cap) {
 template <int TS, bool MULTI>
 static void sf_launch(const StepArgs& a, bool adaptive, hipStream_t s, const hipEvent_t* ev) {
     dim3 blk(WG);
-    if (adaptive) {
+    if (adaptive && (a.flags & AMPS_FLAG_NO_SKIP)) {
+        // the FP64 tiles, then every low tile (no hold sweep)
+        constexpr auto kh = sf_high_kernel<TS, true, MULTI>;
+        constexpr auto kl = sf_lowfull_kernel<TS, MULTI>;
+        launch_sweep(kh, dim3(resident_grid<kh>(a.total_tiles, a.grid_cap)), blk, s, ev, a, a.inst, a.tile_inst,
+                     a.worklist, a.wl_count);
+        launch_sweep(kl, dim3(resident_grid<kl>(a.total_tiles, a.grid_cap)), blk, s, ev ? ev + 2 : nullptr, a, a.inst,
+                     a.tile_inst, a.wl_low, a.wl_low_count);
+    } else if (adaptive) {
         // bulk-hold sweep first (it lists the low tiles it does not hold),
         // then FP64 tiles and those low tiles in one launch
-        constexpr auto km = sf_mixed_kernel<TS, MULTI>;
+        constexpr auto km = sf_mixed_kernel<TS, MULTI, false>;
         if (a.t_halo == 2) {
             constexpr auto kb = sf_hold_kernel<TS, 2, MULTI>;
             launch_sweep(kb, dim3(resident_grid<kb, 64>(a.total_tiles, 0)), dim3(64), s, ev, a, a.inst, a.tile_inst,

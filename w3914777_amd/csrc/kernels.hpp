@@ -26,8 +26,8 @@ int launch_monitor_last_init(const StepArgs& a, int tile, int ngroups_eval, int 
                              double* last, hipStream_t s);
 // single: one instance (descriptor 0 for every tile).
 int launch_classify(const InstDesc* inst, const int* tile_inst, int total_tiles, int ngroups, int broadcast,
-                    int single, const double* md, double thr, int* type_old, int* type, unsigned long long* stats,
-                    hipStream_t s);
+                    int single, const double* md, double thr, int dilate, int* type_old, int* type,
+                    unsigned long long* stats, hipStream_t s);
 int launch_gather(const InstDesc* inst, const Buf& b, long long total_cells, int total_tiles, const int* type,
                   int group, int instid, int ntiles_inst, int field, double* dst, int tile, hipStream_t s);
 int launch_nonfinite(const double* v, long long n, unsigned long long* cnt, hipStream_t s);

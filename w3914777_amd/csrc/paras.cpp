@@ -50,6 +50,15 @@ struct Parser {
             } else return a;
         }
     }
+    // An integer division by zero is a parse error (C leaves it undefined).
+    Val divide_checked(Val a, Val b) {
+        if (a.is_int && b.is_int && b.v == 0) {
+            ok = false;
+            g_parse_error = "integer division by zero";
+            return {};
+        }
+        return arith(a, b, '/');
+    }
     Val term() {
         Val a = factor();
         for (;;) {
@@ -57,7 +66,7 @@ struct Parser {
             if (i < s.size() && (s[i] == '*' || s[i] == '/')) {
                 char op = s[i++];
                 Val b = factor();
-                a = arith(a, b, op);
+                a = op == '/' ? divide_checked(a, b) : arith(a, b, op);
             } else return a;
         }
     }
@@ -112,7 +121,7 @@ struct Parser {
             case '-': r.v = a.v - b.v; break;
             case '*': r.v = a.v * b.v; break;
             default:
-                if (r.is_int) r.v = b.v != 0 ? (double)((long long)a.v / (long long)b.v) : 0;
+                if (r.is_int) r.v = (double)((long long)a.v / (long long)b.v);  // b != 0: divide_checked
                 else r.v = a.v / b.v;
         }
         return r;
