@@ -146,9 +146,10 @@ typedef struct {
     double hi_frac_last;     /* fraction of FP64 tiles after the last monitor pass */
     double hi_frac_mean;     /* mean over monitor passes                            */
     double cells;            /* grid points per step, summed over instances        */
-    int64_t fp64_tile_updates;     /* snowflake adaptive steps since amps_init: FP64 tiles
-                                      updated by the mixed sweep, summed over steps    */
-    int64_t low_full_tile_updates; /* ... and low tiles outside the bulk-hold state    */
+    int64_t fp64_tile_updates;     /* snowflake low-precision steps since amps_init: FP64
+                                      tiles updated, summed over steps                */
+    int64_t low_full_tile_updates; /* ... and low tiles given the full update (without
+                                      NO_SKIP: those outside the bulk-hold state)     */
 } amps_info;
 
 /* ---- configuration ---- */
@@ -237,6 +238,16 @@ int amps_set_tile_types(amps_ctx* ctx, int inst, const int32_t* src, size_t n);
  * n == ntiles * type_groups. */
 int amps_get_monitor_values(amps_ctx* ctx, int inst, double* dst, size_t n);
 
+/* Raw low-precision state of field `field` of instance `inst`: the fp16
+ * words of every cell ([ny][nx], meaningful on tiles whose type is 1) and
+ * each tile's FP64 offset and power-of-two scale ([tiles_y][tiles_x]); a
+ * low tile's value is off + h * scl.  n == nx*ny, ntiles == tiles of the
+ * instance.  Contexts with low-precision storage only.  No reference
+ * counterpart (the reference keeps half2 arrays in managed memory that its
+ * drivers read directly). */
+int amps_get_raw_field(amps_ctx* ctx, int inst, int field, uint16_t* words, size_t n, double* off, float* scl,
+                       size_t ntiles);
+
 int amps_get_info(amps_ctx* ctx, amps_info* info);
 
 /* ---- measurement ---- */
@@ -252,11 +263,13 @@ int amps_set_kernel_timing(amps_ctx* ctx, int enable);
 /* Number of timed sweep-kernel launches and their summed duration (ms). */
 int amps_get_kernel_time(amps_ctx* ctx, int64_t* launches, double* total_ms);
 
-/* The same split by kernel.  The snowflake's adaptive step runs two: first
- * the bulk-hold sweep over the low tiles (AMPS_SWEEP_LOW), then one kernel
- * that updates the FP64 tiles together with the low tiles outside the
- * bulk-hold state (AMPS_SWEEP_FP64).  For every other step AMPS_SWEEP_FP64
- * is the whole sweep and AMPS_SWEEP_LOW is zero.  Replaces the per-kernel
+/* The same split by kernel.  The snowflake's low-precision step runs two:
+ * first the bulk-hold sweep over the low tiles (AMPS_SWEEP_LOW), then one
+ * kernel that updates the FP64 tiles together with the low tiles outside the
+ * bulk-hold state (AMPS_SWEEP_FP64); with AMPS_FLAG_NO_SKIP, first the sweep
+ * of the FP64 tiles (AMPS_SWEEP_FP64), then the full sweep of every low tile
+ * (AMPS_SWEEP_LOW).  For every other step AMPS_SWEEP_FP64 is the whole sweep
+ * and AMPS_SWEEP_LOW is zero.  Replaces the per-kernel
  * cudaEvent pairs around kernel1_mix/mix_kernel2
  * (snowflake/monitor_independent_withoutmonitor.cu:85-127). */
 enum { AMPS_SWEEP_ALL = 0, AMPS_SWEEP_FP64 = 1, AMPS_SWEEP_LOW = 2 };

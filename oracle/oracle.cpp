@@ -234,9 +234,15 @@ void gg_step(int nx, int ny, int q, const double* p, const S* pL, const S* pkapp
 // con1_pure, con2_pure, phi1_pure, phi2_pure (sintering/function.h:51-159) in
 // the order of sintering/main.cu:136-146: mu from the old state, c updated in
 // place (clamped), then eta_1 and eta_2 from the NEW c and the OLD etas.
+//
+// cons: the Cahn-Hilliard update in divergence form (no reference
+// counterpart), c' = clamp01(c + dt div(M grad mu)) with the 5-point
+// divergence of face fluxes F = 0.5 (M_left + M_right) (mu_right - mu_left)
+// over the faces between each cell and its four neighbours; mu and M are
+// the reference's cell values.
 template <class S>
 void sn_step(int nx, int ny, const double* p, const S* pp, int bc, const S* con, const S* eta1,
-             const S* eta2, S* con_out, S* eta1_out, S* eta2_out, int threads) {
+             const S* eta2, S* con_out, S* eta1_out, S* eta2_out, int threads, bool cons = false) {
     // pp (may be null): [dsurf, dgb, kc, ke, L, dvol] overriding p
     const double dx = p[AMPS_SN_DX], dy = p[AMPS_SN_DY], dtime = p[AMPS_SN_DT];
     const double A = p[AMPS_SN_A], B = p[AMPS_SN_B];
@@ -275,13 +281,29 @@ void sn_step(int nx, int ny, const double* p, const S* pp, int bc, const S* con,
             const int xs1 = bmap(x - 1, nx, bc), xa1 = bmap(x + 1, nx, bc);
             const size_t c = (size_t)y * nx + x;
             // sintering/function.h:95-101
-            S dummy_lap = (G(dummy, y, xs1) + G(dummy, y, xa1) + G(dummy, ys1, x) + G(dummy, ya1, x) -
-                           4.0 * G(dummy, y, x)) / dxdy;
+            auto mob_at = [&](int yy, int xx) {
+                const size_t k = (size_t)yy * nx + xx;
+                S ck = con[k];
+                S phi = s_pow(ck, 3) * (10.0 - 15.0 * ck + 6.0 * s_pow(ck, 2));
+                S sum = eta1[k] * eta2[k] * 2.0;
+                return dvol * phi + dvap * (1.0 - phi) + dsur * ck * (1.0 - ck) + dgrb * sum;
+            };
             S cc = con[c];
-            S phi = s_pow(cc, 3) * (10.0 - 15.0 * cc + 6.0 * s_pow(cc, 2));
-            S sum = eta1[c] * eta2[c] * 2.0;
-            S mobil = dvol * phi + dvap * (1.0 - phi) + dsur * cc * (1.0 - cc) + dgrb * sum;
-            S cn = cc + dtime * mobil * dummy_lap;
+            S cn;
+            if (!cons) {
+                S dummy_lap = (G(dummy, y, xs1) + G(dummy, y, xa1) + G(dummy, ys1, x) + G(dummy, ya1, x) -
+                               4.0 * G(dummy, y, x)) / dxdy;
+                S mobil = mob_at(y, x);
+                cn = cc + dtime * mobil * dummy_lap;
+            } else {
+                // face (left cell a, right cell b): 0.5 (M_a + M_b) (mu_b - mu_a)
+                auto face = [&](int ya, int xa, int yb, int xb) {
+                    return 0.5 * (mob_at(ya, xa) + mob_at(yb, xb)) * (G(dummy, yb, xb) - G(dummy, ya, xa));
+                };
+                S fe = face(y, x, y, xa1), fw = face(y, xs1, y, x);
+                S fn = face(y, x, ya1, x), fs = face(ys1, x, y, x);
+                cn = cc + dtime * (((fe - fw) + (fn - fs)) / dxdy);
+            }
             if (cn >= S(1.0)) cn = S(1.0);
             else if (cn < S(0.0)) cn = S(0.0);
             con_out[c] = cn;
@@ -452,6 +474,13 @@ void oracle_gg_step(int nx, int ny, int q, const double* p, int stencil, int bc,
 void oracle_sn_step(int nx, int ny, const double* p, int bc, const double* con, const double* eta1,
                     const double* eta2, double* con_out, double* eta1_out, double* eta2_out, int threads) {
     sn_step<double>(nx, ny, p, nullptr, bc, con, eta1, eta2, con_out, eta1_out, eta2_out, threads);
+}
+
+// The same step with the Cahn-Hilliard update in divergence form.
+void oracle_sn_step_conservative(int nx, int ny, const double* p, int bc, const double* con, const double* eta1,
+                                 const double* eta2, double* con_out, double* eta1_out, double* eta2_out,
+                                 int threads) {
+    sn_step<double>(nx, ny, p, nullptr, bc, con, eta1, eta2, con_out, eta1_out, eta2_out, threads, true);
 }
 
 // Multi-step runs (in place), timed with steady_clock; return seconds.

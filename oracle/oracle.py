@@ -33,8 +33,8 @@ GRAD_SN = dict(DSURF=101, DGB=102, KC=103, KE=104, L=105, DVOL=106)
 
 def build(force=False):
     """Compile liboracle.so with the committed Makefile (g++, seconds)."""
-    src = os.path.join(_HERE, "oracle.cpp")
-    if force or not os.path.exists(_LIB) or os.path.getmtime(_LIB) < os.path.getmtime(src):
+    srcs = [os.path.join(_HERE, f) for f in ("oracle.cpp", "low_emulation.cpp")]
+    if force or not os.path.exists(_LIB) or os.path.getmtime(_LIB) < max(os.path.getmtime(f) for f in srcs):
         subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "liboracle.so"], check=True)
     return _LIB
 
@@ -64,6 +64,7 @@ def lib():
         L.oracle_sf_step.argtypes = [I, I, D, I, I, U64, I64, D, D, D, D, I]
         L.oracle_gg_step.argtypes = [I, I, I, D, I, I, D, D, I]
         L.oracle_sn_step.argtypes = [I, I, D, I, D, D, D, D, D, D, I]
+        L.oracle_sn_step_conservative.argtypes = [I, I, D, I, D, D, D, D, D, D, I]
         L.oracle_sf_run.argtypes = [I, I, D, I, I, U64, I64, I, D, D, I]
         L.oracle_sf_run.restype = ctypes.c_double
         L.oracle_gg_run.argtypes = [I, I, I, D, I, I, I, D, I]
@@ -77,6 +78,12 @@ def lib():
         L.oracle_monitor_temporal.argtypes = [I, I, I, D, D, D]
         L.oracle_monitor_gradient.argtypes = [I, I, I, I, ctypes.c_double, ctypes.c_double, D, D]
         L.oracle_get_type.argtypes = [I, I, D, ctypes.c_double, I32P, I32P]
+        U16P = ctypes.POINTER(ctypes.c_uint16)
+        FP = ctypes.POINTER(ctypes.c_float)
+        L.oracle_sf_low_step.argtypes = [I, I, I, D, I, I, U64, I64, I32P, D, D, U16P, D, FP, ctypes.c_float,
+                                         U16P, D, FP]
+        L.oracle_h2d.argtypes = [ctypes.c_uint16]
+        L.oracle_h2d.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -226,11 +233,14 @@ def gg_run(eta, p, nsteps, stencil=STENCIL_REF, bc=BC_PERIODIC, threads=0):
     return eta, sec
 
 
-def sn_step(c, e1, e2, p, bc=BC_PERIODIC, threads=0):
+def sn_step(c, e1, e2, p, bc=BC_PERIODIC, threads=0, conservative=False):
+    """One sintering step; conservative: the Cahn-Hilliard update in
+    divergence form (face mobilities 0.5 (M_left + M_right))."""
     ny, nx = c.shape
     c, e1, e2 = (np.ascontiguousarray(a, np.float64) for a in (c, e1, e2))
     co, ao, bo = np.empty_like(c), np.empty_like(c), np.empty_like(c)
-    lib().oracle_sn_step(nx, ny, _d(p), bc, _d(c), _d(e1), _d(e2), _d(co), _d(ao), _d(bo), threads)
+    f = lib().oracle_sn_step_conservative if conservative else lib().oracle_sn_step
+    f(nx, ny, _d(p), bc, _d(c), _d(e1), _d(e2), _d(co), _d(ao), _d(bo), threads)
     return co, ao, bo
 
 
@@ -299,3 +309,58 @@ def get_type(md, thr, types):
     old = np.zeros_like(t)
     lib().oracle_get_type(tx, ty, _d(np.ascontiguousarray(md)), thr, _i32(old), _i32(t))
     return old, t
+
+
+# --------------------------------------------------------- low-tile emulation
+def _u16(a):
+    assert a.dtype == np.uint16 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16))
+
+
+def _f32(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def sf_low_step(types, phi, T, words, off, scl, p, tile, stencil=STENCIL_REF, bc=BC_PERIODIC, seed=1, step=0,
+                noise_skip=0.0):
+    """One step of the low tiles (type 1) of a snowflake grid, emulating the
+    engine's fp16-storage / FP32-arithmetic scheme (low_emulation.cpp).
+    words [2, ny, nx] uint16, off [2, ty, tx] float64, scl [2, ty, tx]
+    float32; phi, T hold the FP64 tiles' values.  Returns (words, off, scl)."""
+    ny, nx = phi.shape
+    types = np.ascontiguousarray(types, np.int32)
+    words = np.ascontiguousarray(words, np.uint16)
+    off = np.ascontiguousarray(off, np.float64)
+    scl = np.ascontiguousarray(scl, np.float32)
+    wo, oo, so = np.empty_like(words), np.empty_like(off), np.empty_like(scl)
+    lib().oracle_sf_low_step(nx, ny, tile, _d(p), stencil, bc, seed, step, _i32(types),
+                             _d(np.ascontiguousarray(phi, np.float64)), _d(np.ascontiguousarray(T, np.float64)),
+                             _u16(words), _d(off), _f32(scl), float(noise_skip), _u16(wo), _d(oo), _f32(so))
+    return wo, oo, so
+
+
+_H2D = None
+
+
+def f16_words_to_f64(words):
+    """Exact values of an array of fp16 words."""
+    global _H2D
+    if _H2D is None:
+        _H2D = np.array([lib().oracle_h2d(i) for i in range(65536)], np.float64)
+    return _H2D[np.asarray(words, np.uint16)]
+
+
+def decode_tiles(words, off, scl, tile):
+    """off + h * scl per cell of a [ny, nx] word array with [ty, tx] codes."""
+    h = f16_words_to_f64(words)
+    return np.repeat(np.repeat(off, tile, 0), tile, 1) + h * np.repeat(np.repeat(scl.astype(np.float64), tile, 0), tile, 1)
+
+
+def f16_ulp_value(words, scl, tile):
+    """Spacing of the fp16 grid at each word's magnitude (the gap to the next
+    larger fp16 value), times its tile's scale: the resolution of a low
+    tile's stored value at that cell."""
+    h = np.abs(f16_words_to_f64(words)).astype(np.float16)
+    gap = np.spacing(h).astype(np.float64)
+    return gap * np.repeat(np.repeat(np.asarray(scl, np.float64), tile, 0), tile, 1)

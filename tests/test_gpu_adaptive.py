@@ -246,3 +246,95 @@ def test_adaptive_sintering_keeps_c_fp64(amps, O):
     got = run_mode(amps, ca, 200)
     for g, r in zip(got, ref):
         assert l2_rms(g, r) < 1e-3
+
+
+# ------------------------------------------------- NO_SKIP and dilation
+@pytest.mark.parametrize("tile", [16, 32])
+def test_noskip_all_high_equals_fp64_bitwise(amps, O, tile):
+    """Adaptive with NO_SKIP and a threshold every tile reaches: the FP64
+    tiles go through the adaptive FP64 sweep and must equal the plain FP64
+    path bit for bit."""
+    n = 128
+    c64, _ = sf_config(amps, O, n, tile=tile, mode=0, workload=True)
+    ca, _ = sf_config(amps, O, n, tile=tile, mode=2, monitor=3, thr=0.0, workload=True)
+    ca.flags = amps.FLAG_NO_SKIP
+    a = run_mode(amps, c64, 60)
+    b = run_mode(amps, ca, 60)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_noskip_updates_every_tile_every_step(amps, O):
+    n, t, steps = 256, 32, 200
+    for dil in (0, 2):
+        c, _ = sf_config(amps, O, n, tile=t, mode=2, monitor=3, thr=0.05, workload=True)
+        c.flags, c.dilate = amps.FLAG_NO_SKIP, dil
+        with amps.Context(c) as ctx:
+            ctx.init()
+            ctx.step(steps)
+            ctx.sync()
+            i = ctx.info()
+            types = ctx.get_tile_types()
+        assert i.fp64_tile_updates + i.low_full_tile_updates == steps * (n // t) ** 2
+        assert 0 < (types == 1).sum() < types.size
+
+
+def test_dilation_grows_the_fp64_set(amps, O):
+    """The FP64 set after a monitor pass with dilation d is the set of tiles
+    within Chebyshev distance d (non-periodic) of a tile the reference rule
+    (own value or a von-Neumann neighbour's >= threshold) makes FP64."""
+    n, t = 256, 16
+    phi, T = smooth_fields(n, n, seed=4, nbump=2)
+    for d in (1, 2):
+        c, p = sf_config(amps, O, n, tile=t, mode=2, monitor=3, thr=0.1, workload=True)
+        c.dilate = d
+        with amps.Context(c) as ctx:
+            ctx.init()
+            ctx.set_field(0, phi)
+            ctx.set_field(1, T)
+            ctx.monitor_now()
+            got = ctx.get_tile_types()
+        md = O.monitor_gradient(phi, t, p[O.SF["DX"]], p[O.SF["DY"]], c.bc)
+        _, base = O.get_type(md, 0.01, np.full(md.shape, 2, np.int32))
+        hi = base == 2
+        exp = np.zeros_like(hi)
+        ty, tx = hi.shape
+        for y, x in zip(*np.nonzero(md >= 0.01)):
+            exp[max(y - d, 0):y + d + 1, max(x - d, 0):x + d + 1] = True
+        assert np.array_equal(got == 2, exp | hi)
+        assert (got == 2).sum() > (base == 2).sum()
+
+
+@pytest.mark.parametrize("flags,dil", [(0, 0), (1, 0), (1, 2)])
+def test_adaptive_batch_instances_equal_single_runs(amps, O, flags, dil):
+    """Three uneven grids (64, 96, 128; different seeds and delta) batched in
+    one adaptive context (work-lists and, without NO_SKIP, hold groups that
+    mix instances) give every instance the fields, tile types and monitor
+    values of the same configuration run alone, bit for bit."""
+    cfgs = []
+    for n, seed, delta in ((64, 3, 0.03), (96, 5, 0.05), (128, 9, 0.07)):
+        c, _ = sf_config(amps, O, n, tile=16, mode=2, monitor=3, thr=0.05, workload=True)
+        c.params[amps.SF["CX"]] = c.params[amps.SF["CY"]] = n // 2
+        c.params[amps.SF["DELTA"]] = delta
+        c.seed, c.flags, c.dilate = seed, flags, dil
+        cfgs.append(c)
+    steps = 120
+    with amps.Context(cfgs) as ctx:
+        ctx.init()
+        ctx.step(steps)
+        ctx.sync()
+        batch = [([ctx.get_field(f, inst=i) for f in range(2)], ctx.get_tile_types(inst=i),
+                  ctx.get_monitor_values(inst=i)) for i in range(3)]
+    mixed = 0
+    for i, c in enumerate(cfgs):
+        with amps.Context(c) as ctx:
+            ctx.init()
+            ctx.step(steps)
+            ctx.sync()
+            single = ([ctx.get_field(f) for f in range(2)], ctx.get_tile_types(), ctx.get_monitor_values())
+        for a, b in zip(batch[i][0], single[0]):
+            assert np.array_equal(a, b), i
+        assert np.array_equal(batch[i][1], single[1]), i
+        assert np.array_equal(batch[i][2], single[2]), i
+        mixed += 0 < (single[1] == 1).sum() < single[1].size
+    assert mixed > 0  # some instance holds both precisions

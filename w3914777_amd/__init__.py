@@ -104,7 +104,8 @@ _lib = None
 
 EXPORTS = ["amps_default_config", "amps_load_paras", "amps_create", "amps_create_batch", "amps_destroy",
            "amps_init", "amps_step", "amps_sync", "amps_monitor_now", "amps_get_field", "amps_set_field",
-           "amps_get_tile_types", "amps_set_tile_types", "amps_get_monitor_values", "amps_get_info",
+           "amps_get_tile_types", "amps_set_tile_types", "amps_get_monitor_values", "amps_get_raw_field",
+           "amps_get_info",
            "amps_set_kernel_timing", "amps_get_kernel_time", "amps_get_sweep_time",
            "amps_tape_begin", "amps_adjoint", "amps_autotune", "amps_status_string", "amps_last_error"]
 
@@ -136,6 +137,8 @@ def lib():
     L.amps_get_tile_types.argtypes = [P, ctypes.c_int, I32P, ctypes.c_size_t]
     L.amps_set_tile_types.argtypes = [P, ctypes.c_int, I32P, ctypes.c_size_t]
     L.amps_get_monitor_values.argtypes = [P, ctypes.c_int, D, ctypes.c_size_t]
+    L.amps_get_raw_field.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint16), ctypes.c_size_t,
+                                     D, ctypes.POINTER(ctypes.c_float), ctypes.c_size_t]
     L.amps_get_info.argtypes = [P, ctypes.POINTER(Info)]
     L.amps_set_kernel_timing.argtypes = [P, ctypes.c_int]
     L.amps_get_kernel_time.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]
@@ -264,6 +267,20 @@ class Context:
         out = np.empty((g, ty, tx), np.float64)
         _check(lib().amps_get_monitor_values(self._h, inst, _dptr(out), out.size))
         return out if g > 1 else out[0]
+
+    def get_raw_field(self, field, inst=0):
+        """(fp16 words [ny, nx] as uint16, tile offsets [ty, tx], tile scales
+        [ty, tx]) of a field with low-precision storage."""
+        ny, nx = self.shape(inst)
+        c = self.cfgs[inst]
+        ty, tx = ny // c.tile, nx // c.tile
+        w = np.empty((ny, nx), np.uint16)
+        off = np.empty((ty, tx), np.float64)
+        scl = np.empty((ty, tx), np.float32)
+        _check(lib().amps_get_raw_field(self._h, inst, field, w.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
+                                        w.size, _dptr(off), scl.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                        off.size))
+        return w, off, scl
 
     def info(self):
         i = Info()

@@ -238,11 +238,12 @@ void derive(InstDesc& d, const amps_config& c) {
 
 bool has_low(const amps_ctx* c) { return c->mode != AMPS_PURE_FP64; }
 
-// The snowflake's low-precision step runs two sweep kernels (bulk-hold sweep,
-// then the mixed sweep) unless NO_SKIP runs every tile in the mixed sweep.
-bool two_sweeps(const amps_ctx* c) {
-    return c->model == AMPS_SNOWFLAKE && has_low(c) && !(c->flags & AMPS_FLAG_NO_SKIP);
-}
+// The snowflake's low-precision step runs two sweep kernels: the bulk-hold
+// sweep over the low tiles, then the mixed sweep (FP64 tiles and the low
+// tiles it did not hold); with NO_SKIP the FP64 sweep, then the full low
+// sweep.
+bool two_sweeps(const amps_ctx* c) { return c->model == AMPS_SNOWFLAKE && has_low(c); }
+bool low_sweep_first(const amps_ctx* c) { return two_sweeps(c) && !(c->flags & AMPS_FLAG_NO_SKIP); }
 
 StepArgs step_args(amps_ctx* c, int from) {
     StepArgs a{};
@@ -822,6 +823,21 @@ int amps_get_monitor_values(amps_ctx* c, int inst, double* dst, size_t n) {
     return AMPS_OK;
 }
 
+int amps_get_raw_field(amps_ctx* c, int inst, int field, uint16_t* words, size_t n, double* off, float* scl,
+                       size_t ntiles) {
+    if (!c || !words || !off || !scl) return fail(AMPS_E_ARG, "NULL argument");
+    if (inst < 0 || inst >= c->ninst || field < 0 || field >= c->nfields) return fail(AMPS_E_ARG, "instance or field out of range");
+    if (!has_low(c) || c->gm.fg[field] < 0) return fail(AMPS_E_STATE, "the field has no low-precision storage");
+    const InstDesc& d = c->desc[inst];
+    if (n != (size_t)d.nx * d.ny || ntiles != (size_t)d.tx * d.ty) return fail(AMPS_E_ARG, "n or ntiles does not match the instance");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const Buf& b = c->buf[c->cur];
+    HIPCHK(hipMemcpy(words, b.f16 + (size_t)field * c->total_cells + d.cell_base, n * sizeof(uint16_t), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(off, b.off + (size_t)field * c->total_tiles + d.tile_base, ntiles * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(scl, b.scl + (size_t)field * c->total_tiles + d.tile_base, ntiles * sizeof(float), hipMemcpyDeviceToHost));
+    return AMPS_OK;
+}
+
 int amps_get_info(amps_ctx* c, amps_info* info) {
     if (!c || !info) return fail(AMPS_E_ARG, "NULL argument");
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -881,7 +897,7 @@ extern "C" int amps_get_sweep_time(amps_ctx* c, int which, int64_t* launches, do
     RCCHK(kev_flush(c));
     *launches = c->klaunches;
     // the snowflake's adaptive step times the low (bulk-hold) sweep first
-    const bool low_first = two_sweeps(c);
+    const bool low_first = low_sweep_first(c);
     const int part = (which == AMPS_SWEEP_FP64) == low_first ? 1 : 0;
     *total_ms = which == AMPS_SWEEP_ALL ? c->kms : c->kms_part[part];
     return AMPS_OK;
@@ -894,6 +910,8 @@ extern "C" int amps_tape_begin(amps_ctx* c, int ckpt_every) {
     if (c->model != AMPS_SNOWFLAKE && c->model != AMPS_SINTERING)
         return fail(AMPS_E_UNSUPPORTED, "the adjoint covers the snowflake and sintering models");
     if (c->mode != AMPS_PURE_FP64) return fail(AMPS_E_UNSUPPORTED, "the adjoint needs a PURE_FP64 context");
+    if (c->flags & AMPS_FLAG_CH_CONSERVATIVE)
+        return fail(AMPS_E_UNSUPPORTED, "the adjoint covers the reference's Cahn-Hilliard form only");
     tape_clear(c);
     RCCHK(flush_adv(c));
     c->tape_every = ckpt_every;

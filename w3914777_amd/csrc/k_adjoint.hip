@@ -182,17 +182,6 @@ __global__ __launch_bounds__(256) void sf_adj2_kernel(AdjArgs a) {
         const double ca = c * cth0 + s * sth0, sa = s * cth0 - c * sth0;
         const double eps = epsb + epsbd * ca, epsd = mend * sa;
         const double A = eps * epsd;
-        // d theta / d(gx, gy) = (-gy, gx) / |g|^2, evaluated on the rescaled gradient
-        double tx = 0.0, ty = 0.0;
-        const double mx = fmax(fabs(gx), fabs(gy));
-        if (mx != 0.0) {
-            int e;
-            frexp(mx, &e);
-            const double u = ldexp(gx, -e), v = ldexp(gy, -e);
-            const double ir2 = 1.0 / (u * u + v * v);
-            tx = ldexp(-v * ir2, -e);
-            ty = ldexp(u * ir2, -e);
-        }
         const double lamP = inv2dy * (gather_y(mu, x, y, 1, D) - gather_y(mu, x, y, -1, D));
         const double lamQ = -inv2dx * (gather_x(mu, x, y, 1, D) - gather_x(mu, x, y, -1, D));
         const double mue = mu[cl];
@@ -201,8 +190,22 @@ __global__ __launch_bounds__(256) void sf_adj2_kernel(AdjArgs a) {
         const double lame = 2.0 * eps * lamE2 + epsd * lamA;
         const double lamd = eps * lamA;
         const double lamth = lame * epsd + lamd * ((double)n * mend * ca);
-        a.w1[D.cell_base + cl] = A * lamP + lamth * tx;
-        a.w2[D.cell_base + cl] = A * lamQ + lamth * ty;
+        // d theta / d(gx, gy) = (-gy, gx) / |g|^2, evaluated on the rescaled
+        // gradient; the cotangent is multiplied in before the power-of-two
+        // scale 2^-e, which overflows on its own for subnormal gradients (a
+        // zero cotangent then gives 0, not 0 * inf)
+        double thx = 0.0, thy = 0.0;
+        const double mx = fmax(fabs(gx), fabs(gy));
+        if (mx != 0.0) {
+            int e;
+            frexp(mx, &e);
+            const double u = ldexp(gx, -e), v = ldexp(gy, -e);
+            const double ir2 = 1.0 / (u * u + v * v);
+            thx = ldexp(lamth * (-v * ir2), -e);
+            thy = ldexp(lamth * (u * ir2), -e);
+        }
+        a.w1[D.cell_base + cl] = A * lamP + thx;
+        a.w2[D.cell_base + cl] = A * lamQ + thy;
         a.w3[D.cell_base + cl] = eps * eps * mue;
         g[0] += lame * (epsb * ca) + lamd * (-epsb * (double)n * sa);
         g[2] += lame * (1.0 + delta * ca) + lamd * (-(double)n * delta * sa);

@@ -20,16 +20,25 @@
 
 namespace amps {
 
-template <int TS>
+template <int TS, bool CONS>
 struct SnLds {
     static constexpr int LWC = TS + 4, LWE = TS + 2, RW = TS + 2;
     double c[LWC * LWC];
     double e1[LWE * LWE], e2[LWE * LWE];
     double mu[RW * RW];
+    double mob[CONS ? RW * RW : 1];  // divergence form: mobility on the tile and its ring
     NbField nbf[2];
     double red[4];
     float redf[4 * WG / 64];  // eta bounds of a low tile (block_minmax2)
 };
+
+// Mobility M = Dvol h + Dvap (1 - h) + Dsurf c(1-c) + Dgb 2 eta1 eta2,
+// h = c^3 (10 - 15c + 6c^2) (sintering/function.h:97-99).
+__device__ inline double sn_mobility(double cv, double a1, double a2, double dvol, double dvap, double dsur,
+                                     double dgb) {
+    const double h = cv * cv * cv * (10.0 - 15.0 * cv + 6.0 * (cv * cv));
+    return dvol * h + dvap * (1.0 - h) + dsur * cv * (1.0 - cv) + dgb * (a1 * a2 * 2.0);
+}
 
 __device__ inline int ring_src_sn(int r, int o, int n, int bc) {
     if (bc == AMPS_BC_PERIODIC) return r;
@@ -49,11 +58,17 @@ __device__ inline double clamp01_ref(double v) {
 // Read-only metadata (descriptors, tile -> instance map, tile types) arrive
 // as restrict-qualified arguments so that the per-tile reads go through the
 // scalar cache; with MULTI = false the descriptor is loop-invariant.
-template <int TS, bool ADAPT, bool MULTI>
+//
+// CONS: the Cahn-Hilliard update in divergence form, c' = clamp01(c + dt
+// div(M grad mu)), with M also evaluated on the ring and face fluxes
+// F = 0.5 (M_left + M_right) (mu_right - mu_left); a face shared by two cells
+// (in two tiles too) gives both the bitwise same flux, so the sum of c is
+// conserved up to the clamp and the final roundings.
+template <int TS, bool ADAPT, bool MULTI, bool CONS>
 __global__ __launch_bounds__(256) void sn_step_kernel(StepArgs a, const InstDesc* __restrict__ inst,
                                                       const int* __restrict__ tinst, const int* __restrict__ types) {
     constexpr int LWC = TS + 4, LWE = TS + 2, RW = TS + 2, PER = TS * TS / WG;
-    __shared__ SnLds<TS> L;
+    __shared__ SnLds<TS, CONS> L;
     for (int b = blockIdx.x; b < a.nwork; b += gridDim.x) {
         const int w = xcd_remap(b, a.nwork);
         const InstDesc& d = inst[MULTI ? tinst[w] : 0];
@@ -135,6 +150,8 @@ __global__ __launch_bounds__(256) void sn_step_kernel(StepArgs a, const InstDesc
             double om = 1.0 - cv;
             double df = B * (2.0 * cv + 4.0 * sum3 - 6.0 * sum2) - A2 * (cv * cv) * om + A2 * cv * (om * om);
             L.mu[e] = df - hkc * con_lap;
+            if (CONS)
+                L.mob[e] = sn_mobility(cv, a1, a2, d.k[SNK_DVOL], d.k[SNK_DVAP], d.k[SNK_DSURF], d.k[SNK_DGB]);
         }
         __syncthreads();
 
@@ -150,13 +167,24 @@ __global__ __launch_bounds__(256) void sn_step_kernel(StepArgs a, const InstDesc
             int ly = e / TS, lx = e - ly * TS;
             gcell[i] = (size_t)(oy + ly) * d.nx + ox + lx;
             int rc = (ly + 1) * RW + lx + 1;
-            double mu_lap = (L.mu[rc - 1] + L.mu[rc + 1] + L.mu[rc - RW] + L.mu[rc + RW] - 4.0 * L.mu[rc]) * inv;
             double cv = L.c[(ly + 2) * LWC + lx + 2];
             int ce = (ly + 1) * LWE + lx + 1;
             double a1 = L.e1[ce], a2 = L.e2[ce];
-            double h = cv * cv * cv * (10.0 - 15.0 * cv + 6.0 * (cv * cv));
-            double mob = dvol * h + dvap * (1.0 - h) + dsur * cv * (1.0 - cv) + dgb * (a1 * a2 * 2.0);
-            double cn = clamp01_ref(cv + dt * mob * mu_lap);
+            double cn;
+            if (CONS) {
+                // faces (left a, right b): 0.5 (M_a + M_b) (mu_b - mu_a)
+                const double* M = L.mob;
+                const double* U = L.mu;
+                const double fe = 0.5 * (M[rc] + M[rc + 1]) * (U[rc + 1] - U[rc]);
+                const double fw = 0.5 * (M[rc - 1] + M[rc]) * (U[rc] - U[rc - 1]);
+                const double fn = 0.5 * (M[rc] + M[rc + RW]) * (U[rc + RW] - U[rc]);
+                const double fs = 0.5 * (M[rc - RW] + M[rc]) * (U[rc] - U[rc - RW]);
+                cn = clamp01_ref(cv + dt * (((fe - fw) + (fn - fs)) * inv));
+            } else {
+                double mu_lap = (L.mu[rc - 1] + L.mu[rc + 1] + L.mu[rc - RW] + L.mu[rc + RW] - 4.0 * L.mu[rc]) * inv;
+                double mob = sn_mobility(cv, a1, a2, dvol, dvap, dsur, dgb);
+                cn = clamp01_ref(cv + dt * mob * mu_lap);
+            }
             oc[gcell[i]] = cn;
             double sum2 = a1 * a1 + a2 * a2;
             double l1 = (L.e1[ce - 1] + L.e1[ce + 1] + L.e1[ce - LWE] + L.e1[ce + LWE] - 4.0 * a1) * inv;
@@ -206,12 +234,18 @@ __global__ __launch_bounds__(256) void sn_step_kernel(StepArgs a, const InstDesc
     }
 }
 
+template <int TS, bool ADAPT, bool CONS>
+static void sn_launch2(const StepArgs& a, int grid, hipStream_t s, const hipEvent_t* ev) {
+    if (a.ninst > 1)
+        launch_sweep(sn_step_kernel<TS, ADAPT, true, CONS>, dim3(grid), dim3(WG), s, ev, a, a.inst, a.tile_inst, a.type);
+    else
+        launch_sweep(sn_step_kernel<TS, ADAPT, false, CONS>, dim3(grid), dim3(WG), s, ev, a, a.inst, a.tile_inst, a.type);
+}
+
 template <int TS, bool ADAPT>
 static void sn_launch(const StepArgs& a, int grid, hipStream_t s, const hipEvent_t* ev) {
-    if (a.ninst > 1)
-        launch_sweep(sn_step_kernel<TS, ADAPT, true>, dim3(grid), dim3(WG), s, ev, a, a.inst, a.tile_inst, a.type);
-    else
-        launch_sweep(sn_step_kernel<TS, ADAPT, false>, dim3(grid), dim3(WG), s, ev, a, a.inst, a.tile_inst, a.type);
+    if (a.flags & AMPS_FLAG_CH_CONSERVATIVE) sn_launch2<TS, ADAPT, true>(a, grid, s, ev);
+    else sn_launch2<TS, ADAPT, false>(a, grid, s, ev);
 }
 
 int launch_sn_step(const StepArgs& a, int tile, bool adaptive, int grid, hipStream_t s, const hipEvent_t* ev) {

@@ -945,6 +945,8 @@ __global__ __launch_bounds__(256, 3) void sf_high_kernel(StepArgs a, const InstD
     const long long step = *a.step_ctr + a.step_add;
     const int n = wlc ? *wlc : a.nwork, G = gridDim.x;
     int b = blockIdx.x;
+    // tiles updated (adaptive steps; one writer per launch)
+    if (ADAPT && a.stats && b == 0 && threadIdx.x == 0) a.stats[2] += (unsigned long long)n;
     if (b >= n) return;
     int w = wl ? wl[xcd_remap(b, n)] : xcd_remap(b, n);
     int tcur = ADAPT ? nb_type_lane<TS>(a, inst[MULTI ? tinst[w] : 0], w) : 2;
@@ -1064,6 +1066,7 @@ __global__ __launch_bounds__(256, 4) void sf_lowfull_kernel(StepArgs a, const In
     const long long step = *a.step_ctr + a.step_add;
     const int n = *wlc, G = gridDim.x;
     int b = blockIdx.x;
+    if (b == 0 && threadIdx.x == 0) a.stats[3] += (unsigned long long)n;  // tiles updated (one writer)
     if (b >= n) return;
     // tile k = w (fields in f, codes in slot), tile k+1 = w1 (types t1)
     int w = wl[xcd_remap(b, n)];
