@@ -275,6 +275,24 @@ int amps_get_kernel_time(amps_ctx* ctx, int64_t* launches, double* total_ms);
 enum { AMPS_SWEEP_ALL = 0, AMPS_SWEEP_FP64 = 1, AMPS_SWEEP_LOW = 2 };
 int amps_get_sweep_time(amps_ctx* ctx, int which, int64_t* launches, double* total_ms);
 
+/* ---- stepping without allocation ---- */
+
+/* One PURE_FP64 snowflake step of one grid (cfg: model SNOWFLAKE, mode
+ * PURE_FP64, validated like amps_create) from d_in to d_out, both
+ * caller-owned device buffers of [2][ny][nx] doubles (phi, then T) that must
+ * not overlap; step is the step index (it keys the noise, like the step count
+ * of a context since amps_init).  The work is enqueued on `stream` (a
+ * hipStream_t; NULL: the null stream) and the call returns without waiting.
+ * No reference counterpart: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  It allocates
+ * nothing: the descriptor and step counter the step kernels read go to a
+ * fixed 4 KiB device array of the library, one per GPU, written by a kernel
+ * at each call; grids are limited only as in amps_create.  Calls that
+ * overlap on one GPU (other streams or host threads) race on that array.
+ * The result equals amps_step on a PURE_FP64 context of the same config. */
+int amps_sf_step_fp64_noalloc(const amps_config* cfg, const double* d_in, double* d_out, int64_t step,
+                              void* stream);
+
 /* ---- adjoint (FP64 contexts; snowflake and sintering) ---- */
 
 /* Parameters a gradient can be taken with respect to. */

@@ -107,7 +107,8 @@ EXPORTS = ["amps_default_config", "amps_load_paras", "amps_create", "amps_create
            "amps_get_tile_types", "amps_set_tile_types", "amps_get_monitor_values", "amps_get_raw_field",
            "amps_get_info",
            "amps_set_kernel_timing", "amps_get_kernel_time", "amps_get_sweep_time",
-           "amps_tape_begin", "amps_adjoint", "amps_autotune", "amps_status_string", "amps_last_error"]
+           "amps_tape_begin", "amps_adjoint", "amps_autotune", "amps_status_string", "amps_last_error",
+           "amps_sf_step_fp64_noalloc"]
 
 
 def lib():
@@ -146,6 +147,8 @@ def lib():
     L.amps_tape_begin.argtypes = [P, ctypes.c_int]
     L.amps_adjoint.argtypes = [P, D, I32P, ctypes.c_int, D, D]
     L.amps_autotune.argtypes = [ctypes.POINTER(Config), ctypes.c_int, ctypes.c_int, ctypes.POINTER(Tuning)]
+    L.amps_sf_step_fp64_noalloc.argtypes = [ctypes.POINTER(Config), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                            ctypes.c_void_p]
     L.amps_status_string.argtypes = [ctypes.c_int]
     L.amps_status_string.restype = ctypes.c_char_p
     L.amps_last_error.argtypes = []
@@ -177,6 +180,15 @@ def autotune(cfgs, steps):
     t = Tuning()
     _check(lib().amps_autotune(arr, len(cfgs), int(steps), ctypes.byref(t)))
     return t
+
+
+def sf_step_fp64_noalloc(cfg, d_in, d_out, step, stream=None):
+    """amps_sf_step_fp64_noalloc on device addresses d_in, d_out (integers,
+    e.g. a torch tensor's data_ptr()) of [2, ny, nx] doubles; enqueued on
+    `stream` (a hipStream_t address, None: the null stream)."""
+    c = cfg.copy()
+    _check(lib().amps_sf_step_fp64_noalloc(ctypes.byref(c), ctypes.c_void_p(d_in), ctypes.c_void_p(d_out),
+                                           int(step), ctypes.c_void_p(stream) if stream else None))
 
 
 def tuned(cfg, tuning):

@@ -236,6 +236,19 @@ void derive(InstDesc& d, const amps_config& c) {
     for (int i = 0; i < 24; ++i) d.kf[i] = (float)d.k[i];
 }
 
+// Descriptor of one instance of a validated config (tile base and cell base 0).
+void fill_desc(InstDesc& d, const amps_config& c) {
+    std::memset(&d, 0, sizeof(d));
+    d.nx = c.nx;
+    d.ny = c.ny;
+    d.tx = c.nx / c.tile;
+    d.ty = c.ny / c.tile;
+    d.bc = c.bc;
+    d.stencil = c.stencil;
+    d.seed = c.seed;
+    derive(d, c);
+}
+
 bool has_low(const amps_ctx* c) { return c->mode != AMPS_PURE_FP64; }
 
 // The snowflake's low-precision step runs two sweep kernels: the bulk-hold
@@ -451,6 +464,12 @@ void tape_clear(amps_ctx* c) {
     c->taping = false;
 }
 }  // namespace
+
+namespace amps {
+int validate_config(const amps_config& c) { return validate(c); }
+void config_desc(InstDesc& d, const amps_config& c) { fill_desc(d, c); }
+int fail_with(int code, const char* msg) { return fail(code, msg); }
+}  // namespace amps
 
 // ================================================================ C ABI
 extern "C" {

@@ -14,6 +14,12 @@ struct GroupMap {
 
 enum { CONVERT_CHANGED = 0, CONVERT_ENCODE_LOW = 1, CONVERT_DECODE_LOW = 2 };
 
+// Config validation, the descriptor of one instance (tile base and cell
+// base 0) and error recording, shared with host code outside api.cpp.
+int validate_config(const amps_config& c);
+void config_desc(InstDesc& d, const amps_config& c);
+int fail_with(int code, const char* msg);
+
 int launch_init(const InstDesc* inst_dev, int inst, long long ncells, int model, int init, int q, const Buf& b,
                 long long total_cells, hipStream_t s);
 int launch_convert(const InstDesc* inst, const int* tile_inst, const Buf& b, long long total_cells, int total_tiles,
