@@ -89,13 +89,35 @@ __device__ inline int ring_src(int r, int o, int n, int bc) {
 
 // Phase 1 on a box of R-typed values: P = eps eps' gx, Q = eps eps' gy, E2 = eps^2
 // over the (TS+2)^2 ring box (stored as described at SfGeo::PW).
+//
+// fast: index the ring directly where no ring cell needs the zero-flux
+// mirror (a tile off the domain edge, or periodic boundaries).
 template <int TS, typename Rt>
 __device__ inline void sf_phase1(const Rt* phi, Rt* P, Rt* Q, Rt* E2, const InstDesc& d, int ox, int oy,
-                                 Rt inv2dx, Rt inv2dy, Rt cth0, Rt sth0, Rt epsb, Rt epsbd, Rt mend) {
+                                 Rt inv2dx, Rt inv2dy, Rt cth0, Rt sth0, Rt epsb, Rt epsbd, Rt mend,
+                                 bool fast = false) {
     using G = SfGeo<TS>;
     const int n = (int)d.k[SFK_ANISO_N];
     const bool edge = d.bc != AMPS_BC_PERIODIC &&
                       (ox == 0 || oy == 0 || ox + TS == d.nx || oy + TS == d.ny);
+    if (fast && !edge) {
+        // box index (ry0 + 1) LW + rx0 + 1 = e + 2 ry0 + LW + 1, ring index
+        // RB + ry0 PW + rx0 - 1 = e + 2 ry0 + RB - 1 (LW = PW = RW + 2)
+        static_assert(G::LW == G::RW + 2 && G::PW == G::RW + 2, "ring and box rows differ by 2");
+        for (int e = threadIdx.x; e < G::RW * G::RW; e += WG) {
+            const int ry0 = e / G::RW;
+            const int c = e + 2 * ry0 + G::LW + 1, o = e + 2 * ry0 + G::RB - 1;
+            Rt gx = (phi[c + 1] - phi[c - 1]) * inv2dx;
+            Rt gy = (phi[c + G::LW] - phi[c - G::LW]) * inv2dy;
+            Rt eps, epsd;
+            sf_aniso<Rt>(gx, gy, n, cth0, sth0, epsb, epsbd, mend, eps, epsd);
+            Rt A = eps * epsd;
+            P[o] = A * gx;
+            Q[o] = A * gy;
+            E2[o] = eps * eps;
+        }
+        return;
+    }
     for (int e = threadIdx.x; e < G::RW * G::RW; e += WG) {
         const int ry0 = e / G::RW, rx0 = e - ry0 * G::RW;
         int ry = ry0, rx = rx0;
@@ -234,6 +256,59 @@ __device__ inline void sf_halo_map(SfHalo<TS>& hm, const InstDesc& d, int ox, in
 // group lies inside its row).  The loads go straight into the registers
 // they are decoded from, so the branch around a thread's missing element
 // needs no wait at its join.
+// Per-launch constants of a thread for the tiles of a one-instance grid that
+// need no boundary map: its halo elements relative to the tile's first cell
+// (box index, region; region -1: no element), its ring cells for phase 1
+// and, for lanes 0..8, the offset of tile neighbour k = lane in the tile
+// index.  Computing these per tile costs divisions and boundary maps that
+// an interior tile does not need.
+template <int TS>
+struct SfTileConst {
+    using G = SfGeo<TS>;
+    static constexpr int NR = (G::RW * G::RW + WG - 1) / WG;
+    int hrel[G::NHT], hbox[G::NHT], hreg[G::NHT];
+    int nbrel;
+};
+
+template <int TS>
+__device__ inline void sf_tile_const(SfTileConst<TS>& t, const InstDesc& d) {
+    using G = SfGeo<TS>;
+    constexpr int H = G::H;
+#pragma unroll
+    for (int k = 0; k < G::NHT; ++k) {
+        const int h = threadIdx.x + k * WG;
+        int bly = 0, blx = 0;
+        if (h < G::NH) halo_coord<TS>(h, bly, blx);
+        t.hrel[k] = (bly - H) * d.nx + (blx - H);
+        t.hbox[k] = bly * G::LW + blx;
+        t.hreg[k] = h < G::NH ? box_region<TS, H>(bly, blx) : -1;
+    }
+    const int kk = (threadIdx.x & 63) < 9 ? (int)(threadIdx.x & 63) : 8;
+    t.nbrel = (kk / 3 - 1) * d.tx + (kk % 3 - 1);
+}
+
+// The halo map of tile (tile_x, tile_y): from the launch constants when the
+// tile's halo needs no boundary map, else computed.
+template <int TS>
+__device__ inline void sf_halo_of(SfHalo<TS>& hm, const InstDesc& d, int tile_x, int tile_y,
+                                  const SfTileConst<TS>* tc) {
+    using G = SfGeo<TS>;
+    constexpr int H = G::H;
+    const int ox = tile_x * TS, oy = tile_y * TS;
+    const bool interior = ox >= H && oy >= H && ox + TS + H <= d.nx && oy + TS + H <= d.ny;
+    if (tc && interior) {
+        const int org = oy * d.nx + ox;
+#pragma unroll
+        for (int k = 0; k < G::NHT; ++k) {
+            hm.off[k] = tc->hreg[k] >= 0 ? org + tc->hrel[k] : -1;
+            hm.box[k] = tc->hbox[k];
+            hm.reg[k] = tc->hreg[k] >= 0 ? tc->hreg[k] : 4;
+        }
+    } else {
+        sf_halo_map<TS>(hm, d, ox, oy);
+    }
+}
+
 template <int TS, bool ADAPT>
 __device__ inline void sf_halo_load(uint2* hp, uint2* ht, const StepArgs& a, const InstDesc& d, const SfHalo<TS>& hm,
                                     unsigned hmask) {
@@ -381,8 +456,10 @@ struct LowFetch {
     float sclp, sclT;
 };
 
+// tc (may be null): launch constants of a one-instance grid.
 template <int TS>
-__device__ inline void low_fetch(LowFetch<TS>& f, const StepArgs& a, const InstDesc& d, int w, unsigned hmask) {
+__device__ inline void low_fetch(LowFetch<TS>& f, const StepArgs& a, const InstDesc& d, int w, unsigned hmask,
+                                 const SfTileConst<TS>* tc = nullptr) {
     using G = SfGeo<TS>;
     constexpr int R = G::R;
     const int lt = w - d.tile_base;
@@ -402,7 +479,7 @@ __device__ inline void low_fetch(LowFetch<TS>& f, const StepArgs& a, const InstD
         for (int j = 0; j < R; ++j) { f.op[j] = hphi[gcell0 + j]; f.ot[j] = hT[gcell0 + j]; }
     }
     SfHalo<TS> hm;
-    sf_halo_map<TS>(hm, d, ox, oy);
+    sf_halo_of<TS>(hm, d, tile_x, tile_y, tc);
     sf_halo_load<TS, true>(f.hp, f.ht, a, d, hm, hmask);
     f.offp = a.in.off[w];
     f.offT = a.in.off[a.total_tiles + w];
@@ -431,15 +508,26 @@ struct NbCode {
     float s0, s1;
 };
 
+// Tile index of neighbour k = min(lane, 8) of tile w: w + the launch
+// constant for a tile off the tile grid's border (tc given), else mapped.
 template <int TS>
-__device__ inline void nb_code_fetch(NbCode& c, const StepArgs& a, const InstDesc& d, int w) {
-    // every thread loads (threads 9.. repeat neighbour 8, cache hits): no
-    // branch around the loads, so their wait lands where they are used
+__device__ inline int nb_tile_of(const InstDesc& d, int w, const SfTileConst<TS>* tc) {
     const int lt = w - d.tile_base;
     const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
+    if (tc && tile_x > 0 && tile_y > 0 && tile_x < d.tx - 1 && tile_y < d.ty - 1) return w + tc->nbrel;
     Nb nb;
     nb_setup(nb, d, tile_x, tile_y, TS);
-    const int g = nb_tile(nb, threadIdx.x < 9 ? (int)threadIdx.x : 8);
+    const int l = threadIdx.x & 63;
+    return nb_tile(nb, l < 9 ? l : 8);
+}
+
+template <int TS>
+__device__ inline void nb_code_fetch(NbCode& c, const StepArgs& a, const InstDesc& d, int w,
+                                     const SfTileConst<TS>* tc = nullptr) {
+    // every thread loads (threads 9.. repeat neighbour 8, cache hits): no
+    // branch around the loads, so their wait lands where they are used
+    // (threads 0..8 are lanes 0..8 of wave 0)
+    const int g = nb_tile_of<TS>(d, w, tc);
     c.o0 = a.in.off[g];
     c.o1 = a.in.off[a.total_tiles + g];
     c.s0 = a.in.scl[g];
@@ -468,7 +556,8 @@ __device__ inline void nb_code_store(SfLowLds<TS>& L, int slot, const NbCode& c)
 // evaluated, as the reference's type-1 branch does).
 template <int TS, bool NOSKIP>
 __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const StepArgs& a, const InstDesc& d,
-                            int gtile, const LowFetch<TS>& f, unsigned hmask, long long step) {
+                            int gtile, const LowFetch<TS>& f, unsigned hmask, long long step,
+                            const SfTileConst<TS>* tcon = nullptr) {
     using G = SfGeo<TS>;
     constexpr int H = G::H, LW = G::LW, R = G::R;
     const int lt = gtile - d.tile_base;
@@ -485,7 +574,7 @@ __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const 
         tc[j] = __half2float(__ushort_as_half(f.ot[j])) * sclT;
     }
     SfHalo<TS> hm;
-    sf_halo_map<TS>(hm, d, ox, oy);
+    sf_halo_of<TS>(hm, d, tile_x, tile_y, tcon);
     const float c0 = (float)offp, c1 = (float)(1.0 - offp);
     bool near0 = true, near1 = true;
 #pragma unroll
@@ -556,7 +645,7 @@ __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const 
 
     sf_phase1<TS, float>(L.phi, L.P, L.Q, L.E2, d, ox, oy, d.kf[SFK_INV_2DX], d.kf[SFK_INV_2DY],
                          d.kf[SFK_COS_N_THETA0], d.kf[SFK_SIN_N_THETA0], d.kf[SFK_EPSB], d.kf[SFK_EPSB_DELTA],
-                         d.kf[SFK_MEPSB_ANISO_DELTA]);
+                         d.kf[SFK_MEPSB_ANISO_DELTA], tcon != nullptr);
     __syncthreads();  // B
 
     float rp[R], rt[R];
@@ -920,16 +1009,12 @@ __global__ __launch_bounds__(64) void sf_hold_kernel(StepArgs a, const InstDesc*
 // neighbours.  Issued one tile ahead, so the halo loads of a tile never wait
 // on a type load.
 template <int TS>
-__device__ inline int nb_type_lane(const StepArgs& a, const InstDesc& d, int w) {
+__device__ inline int nb_type_lane(const StepArgs& a, const InstDesc& d, int w, const SfTileConst<TS>* tc = nullptr) {
     const int lane = threadIdx.x & 63;
     if (a.type == nullptr) return 2;
-    const int lt = w - d.tile_base;
-    const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
-    Nb nb;
-    nb_setup(nb, d, tile_x, tile_y, TS);
     // every lane loads (lanes 9.. repeat neighbour 8): no branch around the
     // load, so the wait for it can be placed exactly one tile later
-    const int t = a.type[nb_tile(nb, lane < 9 ? lane : 8)];
+    const int t = a.type[nb_tile_of<TS>(d, w, tc)];
     return lane < 9 ? t : 2;
 }
 
@@ -1072,18 +1157,22 @@ __global__ __launch_bounds__(256, 4) void sf_lowfull_kernel(StepArgs a, const In
     int w = wl[xcd_remap(b, n)];
     const bool more1 = b + G < n;
     int w1 = more1 ? wl[xcd_remap(b + G, n)] : w;
+    // one instance: per-launch offsets of the interior tiles
+    SfTileConst<TS> tconst;
+    if (!MULTI) sf_tile_const<TS>(tconst, inst[0]);
+    const SfTileConst<TS>* tc = MULTI ? nullptr : &tconst;
     unsigned hm;
     LowFetch<TS> f;
     {
         const InstDesc& d = inst[MULTI ? tinst[w] : 0];
-        const int t = nb_type_lane<TS>(a, d, w);
+        const int t = nb_type_lane<TS>(a, d, w, tc);
         NbCode c;
-        nb_code_fetch<TS>(c, a, d, w);
+        nb_code_fetch<TS>(c, a, d, w, tc);
         hm = hmask_of(t);
-        low_fetch<TS>(f, a, d, w, hm);
+        low_fetch<TS>(f, a, d, w, hm, tc);
         nb_code_store<TS>(L, 0, c);
     }
-    int t1 = nb_type_lane<TS>(a, inst[MULTI ? tinst[w1] : 0], w1);
+    int t1 = nb_type_lane<TS>(a, inst[MULTI ? tinst[w1] : 0], w1, tc);
     __syncthreads();  // slot 0 visible
     int slot = 0;
     for (; b < n; b += G) {
@@ -1097,17 +1186,17 @@ __global__ __launch_bounds__(256, 4) void sf_lowfull_kernel(StepArgs a, const In
         LowFetch<TS> f1;
         NbCode c1;
         if (more) {
-            low_fetch<TS>(f1, a, d1, w1, hm1);
-            nb_code_fetch<TS>(c1, a, d1, w1);
+            low_fetch<TS>(f1, a, d1, w1, hm1, tc);
+            nb_code_fetch<TS>(c1, a, d1, w1, tc);
         } else {
             f1 = f;
             c1 = NbCode{0.0, 0.0, 0.0f, 0.0f};
         }
-        const int t2 = b2 < n ? nb_type_lane<TS>(a, inst[MULTI ? tinst[w2] : 0], w2) : 2;
+        const int t2 = b2 < n ? nb_type_lane<TS>(a, inst[MULTI ? tinst[w2] : 0], w2, tc) : 2;
         const InstDesc& d = inst[MULTI ? tinst[w] : 0];
         // stores c1 into slot ^ 1 after its first barrier, which its later
         // barriers make visible (no barrier at the end: see sf_tile_low)
-        sf_tile_low<TS, true>(L, slot, c1, a, d, w, f, hm, step);
+        sf_tile_low<TS, true>(L, slot, c1, a, d, w, f, hm, step, tc);
         slot ^= 1;
         w = w1;
         w1 = w2;
