@@ -378,6 +378,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-others", action="store_true", help="skip the grain-growth, sintering and ensemble lines")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="with --profile-run: a 1 GiB device copy first (the calibration kernel of FETCH_SIZE / "
+                         "WRITE_SIZE counter passes)")
     ap.add_argument("--profile-run", action="store_true",
                     help="only the adaptive path's timed windows (for rocprofv3 runs); prints the window's "
                          "algorithmic bytes per launch of each sweep")
@@ -400,6 +403,8 @@ def main():
     workload_key = f"snowflake4096_nsk_d{args.dilate}_h{horizon}_k{K}_r{R}"
 
     if args.profile_run:
+        if args.copy_probe:
+            copy_bandwidth(reps=1)
         r = timed_run(amps, amp_cfg, horizon, K, R, W, fields=False)
         i = r["info"]
         n = i.steps_done
@@ -466,7 +471,7 @@ def main():
     out["adaptive_step_kernels_us"]["tiles_per_step"] = {k: v[2] for k, v in sweeps.items()}
     kf_s = f64["k_fp64_ms"] / max(f64["k_launches"], 1) / 1e3
     fa = B_FP64 * cells / kf_s / 1e9
-    out["fp64"]["roofline"] = {"kernel": "sf_high_kernel<32,false,false>", "avg_launch_us": kf_s * 1e6,
+    out["fp64"]["roofline"] = {"kernel": "sf_f64_tile_kernel<32,false>", "avg_launch_us": kf_s * 1e6,
                                "achieved": fa, "peak": HBM_PEAK_GBS, "frac": fa / HBM_PEAK_GBS,
                                "frac_of_device_copy": fa / copy_gbs}
 

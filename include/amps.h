@@ -125,7 +125,10 @@ typedef struct {
     int32_t bc;             /* amps_bc                                             */
     int32_t init;           /* amps_init_kind                                      */
     int32_t wg_per_cu;      /* sweep workgroups per CU (sets tiles per workgroup);
-                               0 = as many as fit (occupancy maximum)              */
+                               0 = as many as fit (occupancy maximum).  The
+                               adaptive sintering sweep and the PURE_FP64
+                               snowflake sweep run one workgroup per tile and
+                               ignore it                                      */
     int32_t flags;          /* AMPS_FLAG_* bits; 0 = defaults                      */
     int32_t dilate;         /* ADAPTIVE: the FP64 set also takes every tile within
                                this Chebyshev distance (in tiles) of a tile whose

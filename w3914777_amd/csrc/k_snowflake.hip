@@ -581,8 +581,10 @@ __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const 
     for (int j = 0; j < R; ++j) {
         L.phi[(ly + H) * LW + x0 + H + j] = pc[j];
         L.T[(ly + H) * LW + x0 + H + j] = tc[j];
-        near0 = near0 && fabsf(c0 + pc[j]) <= SF_BULK_EPS;
-        near1 = near1 && fabsf(c1 - pc[j]) <= SF_BULK_EPS;
+        if (!NOSKIP) {
+            near0 = near0 && fabsf(c0 + pc[j]) <= SF_BULK_EPS;
+            near1 = near1 && fabsf(c1 - pc[j]) <= SF_BULK_EPS;
+        }
     }
 #pragma unroll
     for (int k = 0; k < G::NHT; ++k) {
@@ -600,18 +602,23 @@ __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const 
                 (float)(L.nbo[slot][1][r] - offT) + __half2float(__ushort_as_half(wt)) * L.nbs[slot][1][r];
         }
         L.phi[hm.box[k]] = vp;
-        near0 = near0 && fabsf(c0 + vp) <= SF_BULK_EPS;
-        near1 = near1 && fabsf(c1 - vp) <= SF_BULK_EPS;
+        if (!NOSKIP) {
+            near0 = near0 && fabsf(c0 + vp) <= SF_BULK_EPS;
+            near1 = near1 && fabsf(c1 - vp) <= SF_BULK_EPS;
+        }
     }
-    {
+    if (!NOSKIP) {
         const unsigned long long b0 = __ballot(near0), b1 = __ballot(near1);
         if ((threadIdx.x & 63) == 0) L.hflag[threadIdx.x >> 6] = (b0 == ~0ull ? 1u : 0u) | (b1 == ~0ull ? 2u : 0u);
     }
     __syncthreads();  // A
-    unsigned fl = L.hflag[0];
+    bool hold = false;
+    if (!NOSKIP) {
+        unsigned fl = L.hflag[0];
 #pragma unroll
-    for (int i = 1; i < WG / 64; ++i) fl &= L.hflag[i];
-    const bool hold = !NOSKIP && fl != 0u;
+        for (int i = 1; i < WG / 64; ++i) fl &= L.hflag[i];
+        hold = fl != 0u;
+    }
     nb_code_store<TS>(L, slot ^ 1, nxt);
     __half* dphi = a.out.f16 + d.cell_base;
     __half* dT = a.out.f16 + a.total_cells + d.cell_base;
@@ -1049,6 +1056,25 @@ __global__ __launch_bounds__(256, 3) void sf_high_kernel(StepArgs a, const InstD
     }
 }
 
+// PURE_FP64 sweep: one workgroup per tile (the grid is the work count).
+// With no loop over tiles nothing stays live from one tile to the next, so
+// the tile fits in 114 VGPRs (a persistent loop keeps the per-tile constants
+// live and needs 138, spilling SGPRs); occupancy is then bound by the LDS
+// boxes at 3 workgroups per CU either way.
+template <int TS, bool MULTI>
+__global__ __launch_bounds__(256) void sf_f64_tile_kernel(StepArgs a, const InstDesc* __restrict__ inst,
+                                                          const int* __restrict__ tinst) {
+    __shared__ __align__(16) SfLds<TS, double> L;
+    const long long step = *a.step_ctr + a.step_add;
+    const int b = blockIdx.x;
+    if (b >= a.nwork) return;
+    const int w = xcd_remap(b, a.nwork);
+    const InstDesc& d = inst[MULTI ? tinst[w] : 0];
+    const int lt = w - d.tile_base;
+    const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
+    sf_tile_f64<TS, false>(L, a, d, tile_x, tile_y, 0x1FFu, step);
+}
+
 // Mixed sweep of the adaptive step: the FP64 tiles (FP64 work-list) and the
 // low tiles that failed the hold sweep's bulk-hold test (its 8 segments of
 // this step's parity) are the items of one persistent launch, so the two
@@ -1253,9 +1279,8 @@ static void sf_launch(const StepArgs& a, bool adaptive, hipStream_t s, const hip
         launch_sweep(km, dim3(resident_grid<km>(a.total_tiles, a.grid_cap)), blk, s, ev ? ev + 2 : nullptr, a, a.inst,
                      a.tile_inst, a.worklist, a.wl_count, (const int*)a.full_seg, (const int*)a.full_cnt);
     } else {
-        constexpr auto kh = sf_high_kernel<TS, false, MULTI>;
-        launch_sweep(kh, dim3(resident_grid<kh>(a.nwork, a.grid_cap)), blk, s, ev, a, a.inst, a.tile_inst,
-                     (const int*)nullptr, (const int*)nullptr);
+        constexpr auto kh = sf_f64_tile_kernel<TS, MULTI>;
+        launch_sweep(kh, dim3(a.nwork), blk, s, ev, a, a.inst, a.tile_inst);
     }
 }
 

@@ -1,6 +1,6 @@
 """Per-kernel summary of a rocprofv3 output directory (its SQLite database).
 
-    python -m w3914777_amd.rocprof_summary <rocprofv3 -d directory> [--kernel SUBSTRING ...]
+    python -m w3914777_amd.rocprof_summary <rocprofv3 -d directory> [--kernel SUBSTRING ...] [--last N]
 
 Prints one line per kernel: dispatches, total and average duration (kernel
 trace), and the mean of every collected counter per dispatch (--pmc runs).
@@ -19,18 +19,19 @@ def short(name):
 
 
 def load(directory):
+    """Per kernel: durations (ns) and per-counter values, in dispatch order."""
     dbs = sorted(glob.glob(os.path.join(directory, "**", "*.db"), recursive=True))
     if not dbs:
         raise SystemExit(f"no rocprofv3 database under {directory}")
     con = sqlite3.connect(dbs[0])
     kernels, counters = collections.defaultdict(list), collections.defaultdict(list)
     try:
-        for n, dur in con.execute("select name, duration from kernels"):
+        for n, dur in con.execute("select name, duration from kernels order by dispatch_id"):
             kernels[short(n)].append(dur)
     except sqlite3.Error:
         pass
     try:
-        for n, c, v in con.execute("select kernel_name, counter_name, value from counters_collection"):
+        for n, c, v in con.execute("select kernel_name, counter_name, value from counters_collection order by dispatch_id"):
             counters[(short(n), c)].append(v)
     except sqlite3.Error:
         pass
@@ -41,8 +42,13 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("directory")
     ap.add_argument("--kernel", action="append", default=[], help="only kernels whose name contains this")
+    ap.add_argument("--last", type=int, default=0,
+                    help="summarise only each kernel's last N dispatches (e.g. the timed windows of a run)")
     args = ap.parse_args(argv)
     kernels, counters = load(args.directory)
+    if args.last > 0:
+        kernels = {k: v[-args.last:] for k, v in kernels.items()}
+        counters = {k: v[-args.last:] for k, v in counters.items()}
     keep = (lambda k: any(s in k for s in args.kernel)) if args.kernel else (lambda k: True)
     if kernels:
         print(f"{'kernel':<58} {'calls':>8} {'total_ms':>10} {'avg_us':>10} {'min_us':>9} {'max_us':>9}")
