@@ -135,10 +135,14 @@ constexpr double GG_FLAT = 1.0e-10;
 // Read-only metadata (descriptors, tile -> instance map, tile types) arrive
 // as restrict-qualified arguments so that the per-tile reads go through the
 // scalar cache; with MULTI = false the descriptor is loop-invariant.
+// ONE: the grid covers the work (one tile per workgroup); the loop then has
+// a trip count of one that the compiler can see, so nothing is kept live
+// across tiles (a loop of unknown trip count keeps every per-tile constant
+// in registers and spills SGPRs into VGPR lanes).
 // One-wave mapping of 16x16 tiles: per field one vector load of the lane's
 // own strip and at most two halo elements (offsets computed once per tile),
 // the next field's loads in flight while the current field is computed.
-template <int TS, bool ADAPT, bool MULTI>
+template <int TS, bool ADAPT, bool MULTI, bool ONE>
 __global__ __launch_bounds__(64, 4) void gg_wave_kernel(StepArgs a, const InstDesc* __restrict__ inst,
                                                      const int* __restrict__ tinst, const int* __restrict__ types) {
     static_assert(TS == 16, "the one-wave mapping covers 16x16 tiles");
@@ -148,7 +152,7 @@ __global__ __launch_bounds__(64, 4) void gg_wave_kernel(StepArgs a, const InstDe
     const int lane = threadIdx.x;
     const int ly = lane >> 2, x0 = (lane & 3) << 2;
     const int nh = lane + 64 < 68 ? 2 : 1;
-    for (int b = blockIdx.x; b < a.nwork; b += gridDim.x) {
+    for (int b = blockIdx.x, it = 0; b < a.nwork && (!ONE || it == 0); b += gridDim.x, ++it) {
         const int w = xcd_remap(b, a.nwork);
         const InstDesc& d = inst[MULTI ? tinst[w] : 0];
         const int lt = w - d.tile_base;
@@ -486,12 +490,12 @@ __device__ inline void gg_load_raw(unsigned long long* raw, const StepArgs& a, c
 // field, the carried sum of squares and field 0's box; afterwards the box of
 // field f+1 is loaded into registers while field f is computed, so each field
 // costs one overlapped load round.
-template <int TS, bool ADAPT, bool MULTI>
+template <int TS, bool ADAPT, bool MULTI, bool ONE>
 __global__ __launch_bounds__(256, 4) void gg_block_kernel(StepArgs a, const InstDesc* __restrict__ inst,
                                                       const int* __restrict__ tinst, const int* __restrict__ types) {
     constexpr int H = 1, LW = TS + 2, R = TS * TS / WG, N = BoxMap<TS, 1>::N;
     __shared__ GgBlockLds<TS> L;
-    for (int b = blockIdx.x; b < a.nwork; b += gridDim.x) {
+    for (int b = blockIdx.x, it = 0; b < a.nwork && (!ONE || it == 0); b += gridDim.x, ++it) {
         const int w = xcd_remap(b, a.nwork);
         const InstDesc& d = inst[MULTI ? tinst[w] : 0];
         const int lt = w - d.tile_base;
@@ -688,19 +692,25 @@ __global__ __launch_bounds__(256) void gg_aux_kernel(StepArgs a, const int* __re
     }
 }
 
-template <int TS, bool ADAPT>
-static void gg_launch(const StepArgs& a, int grid, hipStream_t s, const hipEvent_t* ev) {
+template <int TS, bool ADAPT, bool ONE>
+static void gg_launch3(const StepArgs& a, int grid, hipStream_t s, const hipEvent_t* ev) {
     if constexpr (TS == 16) {
         if (a.ninst > 1)
-            launch_sweep(gg_wave_kernel<TS, ADAPT, true>, dim3(grid), dim3(GW), s, ev, a, a.inst, a.tile_inst, a.type);
+            launch_sweep(gg_wave_kernel<TS, ADAPT, true, ONE>, dim3(grid), dim3(GW), s, ev, a, a.inst, a.tile_inst, a.type);
         else
-            launch_sweep(gg_wave_kernel<TS, ADAPT, false>, dim3(grid), dim3(GW), s, ev, a, a.inst, a.tile_inst, a.type);
+            launch_sweep(gg_wave_kernel<TS, ADAPT, false, ONE>, dim3(grid), dim3(GW), s, ev, a, a.inst, a.tile_inst, a.type);
     } else {
         if (a.ninst > 1)
-            launch_sweep(gg_block_kernel<TS, ADAPT, true>, dim3(grid), dim3(WG), s, ev, a, a.inst, a.tile_inst, a.type);
+            launch_sweep(gg_block_kernel<TS, ADAPT, true, ONE>, dim3(grid), dim3(WG), s, ev, a, a.inst, a.tile_inst, a.type);
         else
-            launch_sweep(gg_block_kernel<TS, ADAPT, false>, dim3(grid), dim3(WG), s, ev, a, a.inst, a.tile_inst, a.type);
+            launch_sweep(gg_block_kernel<TS, ADAPT, false, ONE>, dim3(grid), dim3(WG), s, ev, a, a.inst, a.tile_inst, a.type);
     }
+}
+
+template <int TS, bool ADAPT>
+static void gg_launch(const StepArgs& a, int grid, hipStream_t s, const hipEvent_t* ev) {
+    if (grid >= a.nwork) gg_launch3<TS, ADAPT, true>(a, grid, s, ev);
+    else gg_launch3<TS, ADAPT, false>(a, grid, s, ev);
 }
 
 int launch_gg_step(const StepArgs& a, int tile, bool adaptive, int grid, hipStream_t s, const hipEvent_t* ev) {
