@@ -149,10 +149,12 @@ def copy_bandwidth(reps=10):
         hip.hipFree(b)
 
 
-def traffic_from_profiles(kernel_key, workload_key):
-    """HBM bytes per launch of a kernel (FETCH_SIZE + WRITE_SIZE, calibrated
-    on the copy kernel) from the committed rocprofv3 --pmc summary, used only
-    when it was measured on this exact workload and step window."""
+def traffic_from_profiles(kernel_key, workload_key, start_step, end_step):
+    """HBM bytes per launch of a sweep kernel (FETCH_SIZE + WRITE_SIZE,
+    calibrated on the copy kernel) from the committed rocprofv3 --pmc
+    summary, used only when it was measured on this workload's trajectory
+    and covers exactly the steps of this run's windows (the summary keeps
+    per-block means of the counters of each step's dispatch)."""
     path = os.path.join(ROOT, "profiles", kernel_key + "_traffic.json")
     try:
         with open(path) as f:
@@ -161,7 +163,13 @@ def traffic_from_profiles(kernel_key, workload_key):
         return None
     if t.get("workload_key") != workload_key:
         return None
-    return t
+    first, last, blk = t["first_step"], t["end_step"], t["block_steps"]
+    if not (first <= start_step < end_step <= last) or (start_step - first) % blk or (end_step - start_step) % blk:
+        return None
+    i0, i1 = (start_step - first) // blk, (end_step - first) // blk
+    fb, wb = t["fetch_size_kib_blocks"][i0:i1], t["write_size_kib_blocks"][i0:i1]
+    b = (t["fetch_factor"] * sum(fb) / len(fb) + t["write_factor"] * sum(wb) / len(wb)) * 1024.0
+    return {"bytes_per_launch": b, "source": t["source"], "steps": [start_step, end_step]}
 
 
 def cpu_model():
@@ -234,35 +242,43 @@ def other_workloads(amps, args):
                     out["grain"] = (s2, gid)
         return out
 
-    # grain growth
+    # grain growth: the adaptive run at --gg-threshold (NO_SKIP), a frontier
+    # of monitor thresholds, and the engine's skipping variant
     gg, ts = amps.Config.from_paras(amps.GRAINGROWTH, os.path.join(PARAS, "graingrowth", "workload2048.h"))
     cells = gg.nx * gg.ny
+    frontier_thr = [t for t in (1e-4, 1e-3, gg.threshold) if t != args.gg_threshold]
+    plan = [("fp64", amps.PURE_FP64, 0, gg.threshold), ("adaptive", amps.ADAPTIVE, amps.FLAG_NO_SKIP, args.gg_threshold)]
+    plan += [(f"adaptive_thr_{t:g}", amps.ADAPTIVE, amps.FLAG_NO_SKIP, t) for t in frontier_thr]
+    plan += [("adaptive_skipping", amps.ADAPTIVE, 0, gg.threshold)]
     runs = {}
-    for name, mode, flags, dil in (("fp64", amps.PURE_FP64, 0, 0),
-                                   ("adaptive", amps.ADAPTIVE, amps.FLAG_NO_SKIP, args.gg_dilate),
-                                   ("adaptive_d0", amps.ADAPTIVE, amps.FLAG_NO_SKIP, 0),
-                                   ("adaptive_skipping", amps.ADAPTIVE, 0, 0)):
+    for name, mode, flags, thr in plan:
         c = gg.copy()
-        c.mode, c.flags, c.dilate = mode, flags, dil
+        c.mode, c.flags, c.threshold = mode, flags, thr
         t = amps.autotune(c, 50)
         r = whole_run(amps.tuned(c, t), ts)
-        r["tile"], r["wg_per_cu"] = t.tile, t.wg_per_cu
+        r["tile"], r["wg_per_cu"], r["threshold"] = t.tile, t.wg_per_cu, thr
         runs[name] = r
         log(f"[bench] grain growth {name}: {r['wall'] * 1e3 / ts:.3f} ms/step (tile {t.tile}, {t.wg_per_cu} wg/CU)")
     f = runs["fp64"]
     out = {"steps": ts, "cells": cells, "unit": "cell-updates/s",
            "fp64": {"value": cells * ts / f["wall"], "ms_per_step": f["wall"] * 1e3 / ts, "tile": f["tile"]}}
-    for name in ("adaptive", "adaptive_d0", "adaptive_skipping"):
-        a = runs[name]
-        out[name] = {"value": cells * ts / a["wall"], "ms_per_step": a["wall"] * 1e3 / ts, "tile": a["tile"],
-                     "hi_frac_mean": a["info"].hi_frac_mean,
-                     "speedup_amp_over_fp64": f["wall"] / a["wall"],
-                     "rel_l2_eta_fields_0_2": [rel_l2(x, y) for x, y in zip(a["fields"], f["fields"])],
-                     "grain_map_vs_fp64": {
-                         "sum_eta2_rel_l2": rel_l2(a["grain"][0], f["grain"][0]),
-                         "grain_id_agreement": float(np.mean(a["grain"][1] == f["grain"][1]))}}
-    out["adaptive"]["dilate"] = args.gg_dilate
+
+    def summary(a):
+        return {"value": cells * ts / a["wall"], "ms_per_step": a["wall"] * 1e3 / ts, "tile": a["tile"],
+                "threshold": a["threshold"], "hi_frac_mean": a["info"].hi_frac_mean,
+                "speedup_amp_over_fp64": f["wall"] / a["wall"],
+                "rel_l2_eta_fields_0_2": [rel_l2(x, y) for x, y in zip(a["fields"], f["fields"])],
+                "grain_map_vs_fp64": {
+                    "sum_eta2_rel_l2": rel_l2(a["grain"][0], f["grain"][0]),
+                    "grain_id_agreement": float(np.mean(a["grain"][1] == f["grain"][1]))}}
+
+    out["adaptive"] = summary(runs["adaptive"])
     out["adaptive"]["flags"] = "NO_SKIP"
+    out["threshold_frontier"] = [summary(runs[n]) for n, *_ in plan if n.startswith("adaptive_thr_") or n == "adaptive"]
+    out["threshold_frontier"].sort(key=lambda r: r["threshold"])
+    out["adaptive_skipping"] = summary(runs["adaptive_skipping"])
+    out["adaptive_skipping"]["note"] = ("not NO_SKIP: absent grains flushed to exact zeros and grain interiors held; "
+                                        "the FP64 path has no such skipping")
     res["graingrowth_q36_2048"] = out
 
     # sintering
@@ -372,8 +388,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20, help="W: untimed steps at least")
     ap.add_argument("--repeats", type=int, default=5, help="R: timed windows per path; value is the median")
     ap.add_argument("--horizon", type=int, default=None, help="step the last window ends at (workload: 20000)")
-    ap.add_argument("--dilate", type=int, default=3, help="FP64-set dilation of the adaptive path (tiles)")
-    ap.add_argument("--gg-dilate", type=int, default=2, help="FP64-set dilation of the grain-growth run (tiles)")
+    ap.add_argument("--dilate", type=int, default=2, help="FP64-set dilation of the adaptive path (tiles)")
+    ap.add_argument("--gg-threshold", type=float, default=3e-5,
+                    help="gradient-monitor threshold of the grain-growth adaptive run (NO_SKIP)")
     ap.add_argument("--frontier", default="0,1,2,3,4", help="dilations of the error/speed frontier ('' to skip)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -382,8 +399,10 @@ def main():
                     help="with --profile-run: a 1 GiB device copy first (the calibration kernel of FETCH_SIZE / "
                          "WRITE_SIZE counter passes)")
     ap.add_argument("--profile-run", action="store_true",
-                    help="only the adaptive path's timed windows (for rocprofv3 runs); prints the window's "
-                         "algorithmic bytes per launch of each sweep")
+                    help="only one path's trajectory and timed windows (for rocprofv3 runs); prints the tile "
+                         "updates of the run")
+    ap.add_argument("--profile-path", choices=("adaptive", "fp64"), default="adaptive",
+                    help="with --profile-run: the path to run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported (the path is single-GPU).",
@@ -400,15 +419,15 @@ def main():
     amp_cfg.mode, amp_cfg.flags, amp_cfg.dilate = amps.ADAPTIVE, amps.FLAG_NO_SKIP, args.dilate
     f64_cfg = cfg.copy()
     f64_cfg.mode = amps.PURE_FP64
-    workload_key = f"snowflake4096_nsk_d{args.dilate}_h{horizon}_k{K}_r{R}"
+    workload_key = f"snowflake4096_nsk_d{args.dilate}"
 
     if args.profile_run:
         if args.copy_probe:
             copy_bandwidth(reps=1)
-        r = timed_run(amps, amp_cfg, horizon, K, R, W, fields=False)
+        r = timed_run(amps, amp_cfg if args.profile_path == "adaptive" else f64_cfg, horizon, K, R, W, fields=False)
         i = r["info"]
         n = i.steps_done
-        print(json.dumps({"profile_run": workload_key, "steps": n,
+        print(json.dumps({"profile_run": workload_key if args.profile_path == "adaptive" else "fp64", "steps": n,
                           "fp64_tile_updates": i.fp64_tile_updates, "low_tile_updates": i.low_full_tile_updates}))
         return
 
@@ -459,7 +478,8 @@ def main():
     dom = max(sweeps, key=lambda k: sweeps[k][0])
     t_dom, b_dom, n_dom = sweeps[dom]
     achieved = b_dom / t_dom / 1e9
-    tr = traffic_from_profiles("sf_lowfull_kernel" if "low" in dom else "sf_high_kernel_adaptive", workload_key)
+    tr = traffic_from_profiles("sf_lowfull_kernel" if "low" in dom else "sf_high_kernel_adaptive", workload_key,
+                               amp["start_step"], amp["end_step"])
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": achieved / HBM_PEAK_GBS,
                        "traffic": tr["bytes_per_launch"] if tr else None,
