@@ -410,8 +410,9 @@ def main():
     ap.add_argument("--profile-run", action="store_true",
                     help="only one path's trajectory and timed windows (for rocprofv3 runs); prints the tile "
                          "updates of the run")
-    ap.add_argument("--profile-path", choices=("adaptive", "fp64"), default="adaptive",
-                    help="with --profile-run: the path to run")
+    ap.add_argument("--profile-path", choices=("adaptive", "fp64", "sintering"), default="adaptive",
+                    help="with --profile-run: the path to run (sintering: 300 steps of the 1024^2 sintering "
+                         "workload, FP64 then adaptive NO_SKIP)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported (the path is single-GPU).",
@@ -430,6 +431,17 @@ def main():
     f64_cfg.mode = amps.PURE_FP64
     workload_key = f"snowflake4096_nsk_d{args.dilate}"
 
+    if args.profile_run and args.profile_path == "sintering":
+        sn, _ = amps.Config.from_paras(amps.SINTERING, os.path.join(PARAS, "sintering", "workload1024.h"))
+        for mode, flags in ((amps.PURE_FP64, 0), (amps.ADAPTIVE, amps.FLAG_NO_SKIP)):
+            c = sn.copy()
+            c.mode, c.flags = mode, flags
+            with amps.Context(c) as ctx:
+                ctx.init()
+                advance(ctx, 300)
+                print(json.dumps({"profile_run": "sintering1024", "mode": int(mode), "steps": 300,
+                                  "hi_frac_last": ctx.info().hi_frac_last}))
+        return
     if args.profile_run:
         if args.copy_probe:
             copy_bandwidth(reps=1)

@@ -210,9 +210,10 @@ __global__ __launch_bounds__(256) void sn_f64_kernel(StepArgs a, const InstDesc*
 //   vector loads; the etas from the address the tile's type selects (R
 //   doubles, or the aligned fp16 group holding the R words);
 // - the rings (c: 2 cells, etas: 1 cell) are one element per thread (threads
-//   past a ring repeat element 0 and store nothing); a ring eta is read from
-//   both storages together with its neighbour tile's type, offsets and
-//   scales, and decoded in registers.
+//   past a ring repeat element 0 and store nothing); a ring eta's neighbour
+//   tile type, offsets and scales are loaded first, and the eta itself from
+//   the storage that type selects once it arrives (the own cells and the c
+//   ring are in flight meanwhile), then decoded in registers.
 // One barrier then separates staging from phase 1, as in the FP64 sweep.
 template <int TS, bool MULTI, bool CONS>
 __global__ __launch_bounds__(256) void sn_amp_kernel(StepArgs a, const InstDesc* __restrict__ inst,
@@ -239,6 +240,27 @@ __global__ __launch_bounds__(256) void sn_amp_kernel(StepArgs a, const InstDesc*
     const double* f2 = a.in.f64 + 2 * a.total_cells + d.cell_base;
     const unsigned short* h1 = reinterpret_cast<const unsigned short*>(a.in.f16 + a.total_cells + d.cell_base);
     const unsigned short* h2 = reinterpret_cast<const unsigned short*>(a.in.f16 + 2 * a.total_cells + d.cell_base);
+    // ---- the ring eta element's neighbour tile: its type, offsets and scales
+    // first (the ring eta load below waits for the type alone: loads return
+    // in issue order), then everything that does not depend on them
+    const bool edge = ox < 2 || oy < 2 || ox + TS + 2 > d.nx || oy + TS + 2 > d.ny;
+    int reb, regi, nty;
+    double no1, no2;
+    float ns1, ns2;
+    {
+        int bly = 0, blx = 0;
+        sn_ring_coord<TS, 1>(threadIdx.x < RNE ? (int)threadIdx.x : 0, bly, blx);
+        reb = threadIdx.x < RNE ? bly * LWE + blx : -1;
+        regi = edge ? sn_box_cell<1>(d, ox, oy, bly, blx) : (oy - 1 + bly) * d.nx + ox - 1 + blx;
+        Nb nb;
+        nb_setup(nb, d, tile_x, tile_y, TS);
+        const int g = nb_tile(nb, box_region<TS, 1>(bly, blx));
+        nty = types[g];
+        no1 = a.in.off[T + g];
+        no2 = a.in.off[2 * T + g];
+        ns1 = a.in.scl[T + g];
+        ns2 = a.in.scl[2 * T + g];
+    }
     // ---- own cells
     double oc[R];
     if (R == 4) {
@@ -267,8 +289,7 @@ __global__ __launch_bounds__(256) void sn_amp_kernel(StepArgs a, const InstDesc*
     }
     const double oo1 = a.in.off[T + w], oo2 = a.in.off[2 * T + w];
     const float os1 = a.in.scl[T + w], os2 = a.in.scl[2 * T + w];
-    // ---- rings (boxes crossing the grid edge need the boundary map)
-    const bool edge = ox < 2 || oy < 2 || ox + TS + 2 > d.nx || oy + TS + 2 > d.ny;
+    // ---- c ring (boxes crossing the grid edge need the boundary map)
     double rc[KC];
     int rcb[KC];
 #pragma unroll
@@ -280,28 +301,12 @@ __global__ __launch_bounds__(256) void sn_amp_kernel(StepArgs a, const InstDesc*
         const int gi = edge ? sn_box_cell<2>(d, ox, oy, bly, blx) : (oy - 2 + bly) * d.nx + ox - 2 + blx;
         rc[k] = fc[gi];
     }
-    int reb, regi, nty;
-    uint2 r1, r2, s1, s2;  // ring eta: FP64 words (r) and the fp16 group (s)
-    double no1, no2;
-    float ns1, ns2;
-    {
-        int bly = 0, blx = 0;
-        sn_ring_coord<TS, 1>(threadIdx.x < RNE ? (int)threadIdx.x : 0, bly, blx);
-        reb = threadIdx.x < RNE ? bly * LWE + blx : -1;
-        regi = edge ? sn_box_cell<1>(d, ox, oy, bly, blx) : (oy - 1 + bly) * d.nx + ox - 1 + blx;
-        r1 = *reinterpret_cast<const uint2*>(f1 + regi);
-        r2 = *reinterpret_cast<const uint2*>(f2 + regi);
-        s1 = *reinterpret_cast<const uint2*>(h1 + (regi & ~3));
-        s2 = *reinterpret_cast<const uint2*>(h2 + (regi & ~3));
-        Nb nb;
-        nb_setup(nb, d, tile_x, tile_y, TS);
-        const int g = nb_tile(nb, box_region<TS, 1>(bly, blx));
-        nty = types[g];
-        no1 = a.in.off[T + g];
-        no2 = a.in.off[2 * T + g];
-        ns1 = a.in.scl[T + g];
-        ns2 = a.in.scl[2 * T + g];
-    }
+    // ---- eta ring: only the storage the neighbour's type says is current
+    // (the other copy is stale, and a ring column reads a whole memory line
+    // per element)
+    const bool n64 = nty == 2;
+    const uint2 r1 = *(n64 ? reinterpret_cast<const uint2*>(f1 + regi) : reinterpret_cast<const uint2*>(h1 + (regi & ~3)));
+    const uint2 r2 = *(n64 ? reinterpret_cast<const uint2*>(f2 + regi) : reinterpret_cast<const uint2*>(h2 + (regi & ~3)));
     // ---- box staging
     {
         double* cp = &L.c[(sly + 2) * LWC + sx0 + 2];
@@ -329,12 +334,12 @@ __global__ __launch_bounds__(256) void sn_amp_kernel(StepArgs a, const InstDesc*
             }
         }
         if (reb >= 0) {
-            if (nty == 2) {
+            if (n64) {
                 L.e1[reb] = raw_f64(r1.x, r1.y);
                 L.e2[reb] = raw_f64(r2.x, r2.y);
             } else {
-                L.e1[reb] = no1 + (double)__half2float(__ushort_as_half((unsigned short)raw_word4(s1, regi))) * (double)ns1;
-                L.e2[reb] = no2 + (double)__half2float(__ushort_as_half((unsigned short)raw_word4(s2, regi))) * (double)ns2;
+                L.e1[reb] = no1 + (double)__half2float(__ushort_as_half((unsigned short)raw_word4(r1, regi))) * (double)ns1;
+                L.e2[reb] = no2 + (double)__half2float(__ushort_as_half((unsigned short)raw_word4(r2, regi))) * (double)ns2;
             }
         }
     }
