@@ -410,9 +410,10 @@ def main():
     ap.add_argument("--profile-run", action="store_true",
                     help="only one path's trajectory and timed windows (for rocprofv3 runs); prints the tile "
                          "updates of the run")
-    ap.add_argument("--profile-path", choices=("adaptive", "fp64", "sintering"), default="adaptive",
+    ap.add_argument("--profile-path", choices=("adaptive", "fp64", "sintering", "graingrowth"), default="adaptive",
                     help="with --profile-run: the path to run (sintering: 300 steps of the 1024^2 sintering "
-                         "workload, FP64 then adaptive NO_SKIP)")
+                         "workload, FP64 then adaptive NO_SKIP; graingrowth: the Q=36 2048^2 workload's "
+                         "10000 steps, FP64 on 32^2 tiles then adaptive NO_SKIP at --gg-threshold on 16^2)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported (the path is single-GPU).",
@@ -431,6 +432,20 @@ def main():
     f64_cfg.mode = amps.PURE_FP64
     workload_key = f"snowflake4096_nsk_d{args.dilate}"
 
+    if args.profile_run and args.profile_path == "graingrowth":
+        gg, ts = amps.Config.from_paras(amps.GRAINGROWTH, os.path.join(PARAS, "graingrowth", "workload2048.h"))
+        for mode, tile, thr, flags in ((amps.PURE_FP64, 32, gg.threshold, 0),
+                                       (amps.ADAPTIVE, 16, args.gg_threshold, amps.FLAG_NO_SKIP)):
+            c = gg.copy()
+            c.mode, c.tile, c.threshold, c.flags = mode, tile, thr, flags
+            with amps.Context(c) as ctx:
+                ctx.init()
+                t0 = time.perf_counter()
+                advance(ctx, ts)
+                print(json.dumps({"profile_run": "graingrowth2048_q36", "mode": int(mode), "tile": tile,
+                                  "threshold": thr, "steps": ts, "us_per_step": (time.perf_counter() - t0) * 1e6 / ts,
+                                  "hi_frac_mean": ctx.info().hi_frac_mean}))
+        return
     if args.profile_run and args.profile_path == "sintering":
         sn, _ = amps.Config.from_paras(amps.SINTERING, os.path.join(PARAS, "sintering", "workload1024.h"))
         for mode, flags in ((amps.PURE_FP64, 0), (amps.ADAPTIVE, amps.FLAG_NO_SKIP)):
