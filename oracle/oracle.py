@@ -84,6 +84,8 @@ def lib():
                                          U16P, D, FP]
         L.oracle_h2d.argtypes = [ctypes.c_uint16]
         L.oracle_h2d.restype = ctypes.c_double
+        L.oracle_f2h.argtypes = [ctypes.c_float]
+        L.oracle_f2h.restype = ctypes.c_uint16
         _lib = L
     return _lib
 
@@ -364,3 +366,47 @@ def f16_ulp_value(words, scl, tile):
     h = np.abs(f16_words_to_f64(words)).astype(np.float16)
     gap = np.spacing(h).astype(np.float64)
     return gap * np.repeat(np.repeat(np.asarray(scl, np.float64), tile, 0), tile, 1)
+
+
+def _f32_down(x):
+    f = np.float32(x)
+    return np.nextafter(f, np.float32(-np.inf)) if float(f) > x else f
+
+
+def _f32_up(x):
+    f = np.float32(x)
+    return np.nextafter(f, np.float32(np.inf)) if float(f) < x else f
+
+
+def sn_encode_low(v, types, tile):
+    """The engine's re-encoding of a sintering eta field on its low tiles
+    (types == 1), a restatement of its low-tile arithmetic (no reference
+    counterpart: the reference's type-1 tiles compute in half2): the tile's
+    bounds are the minimum rounded down and the maximum rounded up to float;
+    off = (lo + hi) / 2 and hw = (hi - lo) / 2 in double; e is frexp's
+    exponent of hw (-100 for hw = 0, clamped at -100); each word is the RNE
+    fp16 of float(v - off) * 2^(14 - e); the scale is 2^(e - 14) (0 for
+    hw = 0).  Returns (words [ny, nx], off [ty, tx], scl [ty, tx]); FP64
+    tiles are left zero."""
+    L = lib()
+    ny, nx = v.shape
+    ty, tx = ny // tile, nx // tile
+    words = np.zeros((ny, nx), np.uint16)
+    off = np.zeros((ty, tx))
+    scl = np.zeros((ty, tx), np.float32)
+    for j in range(ty):
+        for i in range(tx):
+            if types[j, i] != 1:
+                continue
+            b = v[j * tile:(j + 1) * tile, i * tile:(i + 1) * tile]
+            lo, hi = float(_f32_down(b.min())), float(_f32_up(b.max()))
+            o, hw = 0.5 * (lo + hi), 0.5 * (hi - lo)
+            e = np.frexp(hw)[1] if hw > 0.0 else -100
+            e = max(int(e), -100)
+            invs = np.float32(2.0 ** (14 - e))
+            x = (b - o).astype(np.float32) * invs
+            words[j * tile:(j + 1) * tile, i * tile:(i + 1) * tile] = np.array(
+                [L.oracle_f2h(float(t)) for t in x.ravel()], np.uint16).reshape(b.shape)
+            off[j, i] = o
+            scl[j, i] = 0.0 if hw == 0.0 else np.float32(2.0 ** (e - 14))
+    return words, off, scl

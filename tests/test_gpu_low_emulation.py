@@ -9,7 +9,12 @@ and the
 FP64 tiles must match the FP64 oracle step of the decoded state to 1e-12.
 Steps are chosen between monitor passes, so the tile types do not change.
 The reference branch this stands in for is snowflake/function.h:417-537
-(the type-1 half2 update), whose own arithmetic cannot be run here."""
+(the type-1 half2 update), whose own arithmetic cannot be run here.
+
+Sintering's low tiles (the etas in fp16; c and every update in FP64) are
+checked the same way against the FP64 oracle step of the decoded state and
+the restated re-encoding (oracle.sn_encode_low); reference branches
+sintering/function.h:161-774 (the mixed kernels)."""
 import numpy as np
 import pytest
 
@@ -85,3 +90,60 @@ def test_low_tiles_match_cpu_emulation(amps, O, mode, n, tile, stencil, bc, dil,
                 assert rel_err(phi1[hi], po[hi]) <= 1e-12
                 assert rel_err(T1[hi], to[hi]) <= 1e-12
         assert n_low > 0
+
+
+@pytest.mark.parametrize("bc,tile,cons", [(0, 32, False), (1, 16, False), (1, 32, True)])
+def test_sintering_low_eta_match_cpu_emulation(amps, O, bc, tile, cons):
+    """Adaptive sintering: c is FP64 on every tile and the etas of low tiles
+    are fp16 with offset and scale (sn_amp_kernel).  From the decoded state,
+    one step must give c and the FP64 tiles' etas within 1e-12 of the FP64
+    oracle step, and the low tiles' new words, offsets and scales must
+    decode within one fp16 ulp (times the tile's scale) plus the offsets'
+    difference of the oracle step re-encoded by the restated encoding
+    (oracle.sn_encode_low), for three steps between monitor passes."""
+    import os
+    n = 256
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    cfg, _ = amps.Config.from_paras(amps.SINTERING, os.path.join(root, "w3914777_amd", "paras", "sintering",
+                                                                 "workload1024.h"))
+    cfg.nx = cfg.ny = n
+    cfg.tile, cfg.bc = tile, bc
+    cfg.params[O.SN["R1"]] = cfg.params[O.SN["R2"]] = 50.0
+    cfg.params[O.SN["X1"]], cfg.params[O.SN["X2"]] = 78.0, 178.0
+    cfg.params[O.SN["Y1"]] = cfg.params[O.SN["Y2"]] = 128.0
+    cfg.mode, cfg.flags = amps.ADAPTIVE, amps.FLAG_NO_SKIP | (amps.FLAG_CH_CONSERVATIVE if cons else 0)
+    p = np.array(cfg.params[:], np.float64)
+    step0 = 35  # monitor passes at steps 30 and 40: none in between
+    with amps.Context(cfg) as ctx:
+        ctx.init()
+        ctx.step(step0)
+        ctx.sync()
+        n_low = 0
+        for k in range(3):
+            types = ctx.get_tile_types()  # one type map: the etas' storage
+            c, e1, e2 = (ctx.get_field(f) for f in range(3))
+            ctx.step(1)
+            c1 = ctx.get_field(0)
+            raw1 = [ctx.get_raw_field(f) for f in (1, 2)]
+            assert np.array_equal(ctx.get_tile_types(), types)
+            co, eo1, eo2 = O.sn_step(c, e1, e2, p, bc=bc, conservative=cons)
+            assert rel_err(c1, co) <= 1e-12
+            low = np.repeat(np.repeat(types == 1, tile, 0), tile, 1)
+            n_low += int(low.sum())
+            for f, eo, (gw, go, gs) in ((1, eo1, raw1[0]), (2, eo2, raw1[1])):
+                got = ctx.get_field(f)
+                if (~low).any():
+                    assert rel_err(got[~low], eo[~low]) <= 1e-12, (k, f)
+                xw, xo, xs = O.sn_encode_low(eo, types, tile)
+                g = O.decode_tiles(gw, go, gs, tile)
+                e = O.decode_tiles(xw, xo, xs, tile)
+                assert np.array_equal(g[low], got[low])
+                ulp = np.maximum(O.f16_ulp_value(gw, gs, tile), O.f16_ulp_value(xw, xs, tile))
+                doff = np.repeat(np.repeat(np.abs(go - xo), tile, 0), tile, 1)
+                d = np.abs(g - e)
+                tol = 1.001 * ulp + doff + 1e-300
+                assert np.all(d[low] <= tol[low]), (k, f, float(np.max((d / tol)[low])))
+                # nearly every cell agrees to far better than an ulp (a tile
+                # whose bounds came out one float apart shifts all its words)
+                assert np.mean(d[low] <= 0.01 * ulp[low] + doff[low] + 1e-300) > 0.95, (k, f)
+        assert n_low > 0 and n_low < 3 * n * n

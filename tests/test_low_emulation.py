@@ -132,3 +132,40 @@ def test_uniform_tiles_and_fp64_neighbours(O):
     assert es[0][1, 2] == 0.0
     # FP64 tiles are left to the FP64 path (words and codes untouched)
     assert np.array_equal(ew[0][32:, 32:], wp[32:, 32:]) and eo[0][3, 3] == op[3, 3]
+
+
+def test_sintering_eta_encode_restatement(O):
+    """The restated sintering low-eta encoding: per low tile the scale is a
+    power of two, every |h| <= 2^14 (off is the mid-range of float bounds
+    that cover the values), a uniform tile at a float-exact value gets scale
+    0, and decoding gives
+    the value back within half an fp16 ulp (RNE) plus the float rounding of
+    v - off; FP64 tiles are untouched."""
+    rng = np.random.default_rng(4)
+    n, tile = 64, 16
+    v = np.zeros((n, n))
+    for j in range(4):
+        for i in range(4):
+            lo = float(np.float32(rng.uniform(-0.2, 1.0)))  # float-exact: a uniform tile's bounds coincide
+            span = 10.0 ** rng.uniform(-12, 0) if (i + j) % 5 else 0.0
+            v[j * tile:(j + 1) * tile, i * tile:(i + 1) * tile] = lo + span * rng.random((tile, tile))
+    types = np.ones((4, 4), np.int32)
+    types[1, 2] = 2
+    w, off, scl = O.sn_encode_low(v, types, tile)
+    assert scl[1, 2] == 0.0 and off[1, 2] == 0.0 and not np.any(w[16:32, 32:48])
+    h = O.f16_words_to_f64(w)
+    for j in range(4):
+        for i in range(4):
+            if types[j, i] != 1:
+                continue
+            s = np.s_[j * tile:(j + 1) * tile, i * tile:(i + 1) * tile]
+            if np.ptp(v[s]) == 0.0:
+                assert scl[j, i] == 0.0 and not np.any(w[s])
+                continue
+            assert math.frexp(float(scl[j, i]))[0] == 0.5  # a power of two
+            assert np.max(np.abs(h[s])) <= 2.0 ** 14
+    dec = O.decode_tiles(w, off, scl, tile)
+    ulp = O.f16_ulp_value(w, scl, tile)
+    dev = np.abs(v - np.repeat(np.repeat(off, tile, 0), tile, 1))
+    low = np.repeat(np.repeat(types == 1, tile, 0), tile, 1)
+    assert np.all(np.abs(dec - v)[low] <= (0.5 * ulp + 2.0 ** -24 * dev + 1e-300)[low])
