@@ -289,6 +289,98 @@ void oracle_sf_low_step(int nx, int ny, int tile, const double* p, int stencil, 
     }
 }
 
+// One step of the grain-growth low tiles (the 16x16 one-wave sweep's FP32
+// path; each field has its own tile types): for every low tile of field f,
+// the box of deviations from the tile's offset is decoded as above, then per
+// cell, with eta = (float)off + dv and S the carried sum of squares (the
+// decoded fields' squares summed in field order, FP64, rounded to float):
+//   lap (Moore): fma(-9, dv, (cs[x-1] + cs[x]) + cs[x+1]) / (dx dy), cs the
+//                column sums (u + m) + d of the 3x3 block
+//   lap (5-point): fma(-4, dv, ((m[x-1] + m[x+1]) + u) + d) / (dx dy)
+//   others = fma(-eta, eta, S), df = fma(2 gamma eta, others, fma(beta,
+//   eta^2, -alpha) eta), r = fma(-dt L, fma(-kappa, lap, df), dv)
+// (graingrowth/function.h:20-23; the engine writes these fused multiply-adds
+// explicitly), re-centred as above.  A field whose codes
+// are exactly 0 (offset 0, scale 0, low) on the tile and its 8 neighbours
+// stays exactly 0.  Arrays are [q][ny][nx] (fields, words) and [q][ty][tx]
+// (types, offsets, scales).
+void oracle_gg_low_step(int nx, int ny, int q, int tile, const double* p, int stencil, int bc, const int32_t* types,
+                        const double* v, const uint16_t* w, const double* off, const float* scl, uint16_t* w_out,
+                        double* off_out, float* scl_out) {
+    const int TS = tile, tx = nx / TS, ty = ny / TS, nt = tx * ty;
+    const size_t ncell = (size_t)nx * ny;
+    const float invf = (float)(1.0 / (p[AMPS_GG_DX] * p[AMPS_GG_DY])), dtlf = (float)(p[AMPS_GG_DT] * p[AMPS_GG_L]);
+    const float kapf = (float)p[AMPS_GG_KAPPA], alf = (float)p[AMPS_GG_ALPHA], bef = (float)p[AMPS_GG_BETA];
+    const float g2f = (float)(2.0 * p[AMPS_GG_GAMMA]);
+    const bool moore = stencil == AMPS_STENCIL_REF;
+    std::vector<double> S(ncell, 0.0);  // accumulated as the engine does, with fused multiply-adds
+    for (int f = 0; f < q; ++f)
+        for (size_t c = 0; c < ncell; ++c) S[c] = std::fma(v[f * ncell + c], v[f * ncell + c], S[c]);
+    auto zero_code = [&](int f, int t) { return types[f * nt + t] == 1 && off[f * nt + t] == 0.0 && scl[f * nt + t] == 0.0f; };
+    const int BW = TS + 2;
+    std::vector<float> box((size_t)BW * BW), r((size_t)TS * TS);
+    for (int f = 0; f < q; ++f)
+        for (int t = 0; t < nt; ++t) {
+            if (types[f * nt + t] != 1) continue;
+            const int tyi = t / tx, txi = t % tx, ox = txi * TS, oy = tyi * TS;
+            bool zall = true;
+            for (int k = 0; k < 9 && zall; ++k) {
+                const int gy = bmap(k / 3 == 0 ? oy - 1 : (k / 3 == 2 ? oy + TS : oy), ny, bc);
+                const int gx = bmap(k % 3 == 0 ? ox - 1 : (k % 3 == 2 ? ox + TS : ox), nx, bc);
+                zall = zero_code(f, (gy / TS) * tx + gx / TS);
+            }
+            if (zall) {
+                for (int ly = 0; ly < TS; ++ly)
+                    for (int lx = 0; lx < TS; ++lx) w_out[f * ncell + (size_t)(oy + ly) * nx + ox + lx] = 0;
+                off_out[f * nt + t] = 0.0;
+                scl_out[f * nt + t] = 0.0f;
+                continue;
+            }
+            const double own = off[f * nt + t];
+            const float sc = scl[f * nt + t];
+            for (int by = -1; by <= TS; ++by)
+                for (int bx = -1; bx <= TS; ++bx) {
+                    const int gy = bmap(oy + by, ny, bc), gx = bmap(ox + bx, nx, bc);
+                    const size_t c = (size_t)gy * nx + gx;
+                    const int tn = (gy / TS) * tx + gx / TS;
+                    float x;
+                    if (tn == t) x = h2f(w[f * ncell + c]) * sc;
+                    else if (types[f * nt + tn] == 2) x = (float)(v[f * ncell + c] - own);
+                    else x = (float)(off[f * nt + tn] - own) + h2f(w[f * ncell + c]) * scl[f * nt + tn];
+                    box[(size_t)(by + 1) * BW + bx + 1] = x;
+                }
+            float mn = INFINITY, mx = -INFINITY;
+            const float offf = (float)own;
+            for (int ly = 0; ly < TS; ++ly)
+                for (int lx = 0; lx < TS; ++lx) {
+                    const float* u = &box[(size_t)ly * BW + lx];
+                    const float* m = u + BW;
+                    const float* d = m + BW;
+                    float lap;
+                    if (moore) {
+                        const float c0 = (u[0] + m[0]) + d[0], c1 = (u[1] + m[1]) + d[1], c2 = (u[2] + m[2]) + d[2];
+                        lap = std::fmaf(-9.0f, m[1], (c0 + c1) + c2) * invf;
+                    } else {
+                        lap = std::fmaf(-4.0f, m[1], ((m[0] + m[2]) + u[1]) + d[1]) * invf;
+                    }
+                    const float dv = m[1], eta = offf + dv;
+                    const float others = std::fmaf(-eta, eta, (float)S[(size_t)(oy + ly) * nx + ox + lx]);
+                    const float tt = std::fmaf(bef, eta * eta, -alf);
+                    const float df = std::fmaf(g2f * eta, others, tt * eta);
+                    const float rn = std::fmaf(-dtlf, std::fmaf(-kapf, lap, df), dv);
+                    r[(size_t)ly * TS + lx] = rn;
+                    mn = std::fmin(mn, rn);
+                    mx = std::fmax(mx, rn);
+                }
+            const Code cd = tile_code(mn, mx);
+            for (int ly = 0; ly < TS; ++ly)
+                for (int lx = 0; lx < TS; ++lx)
+                    w_out[f * ncell + (size_t)(oy + ly) * nx + ox + lx] = f2h((r[(size_t)ly * TS + lx] - cd.mid) * cd.inv);
+            off_out[f * nt + t] = own + (double)cd.mid;
+            scl_out[f * nt + t] = cd.scl;
+        }
+}
+
 // fp16 word -> double, for the tests' decoding of emulated tiles.
 double oracle_h2d(uint16_t h) { return (double)h2f(h); }
 uint16_t oracle_f2h(float x) { return f2h(x); }

@@ -86,6 +86,7 @@ def lib():
         L.oracle_h2d.restype = ctypes.c_double
         L.oracle_f2h.argtypes = [ctypes.c_float]
         L.oracle_f2h.restype = ctypes.c_uint16
+        L.oracle_gg_low_step.argtypes = [I, I, I, I, D, I, I, I32P, D, U16P, D, FP, U16P, D, FP]
         _lib = L
     return _lib
 
@@ -410,3 +411,20 @@ def sn_encode_low(v, types, tile):
             off[j, i] = o
             scl[j, i] = 0.0 if hw == 0.0 else np.float32(2.0 ** (e - 14))
     return words, off, scl
+
+
+def gg_low_step(types, v, words, off, scl, p, tile, stencil, bc):
+    """One step of the grain-growth low tiles (low_emulation.cpp): types
+    [q, ty, tx] int32, v decoded fields [q, ny, nx], words [q, ny, nx] uint16,
+    off [q, ty, tx], scl [q, ty, tx] float32.  Returns (words, off, scl) of
+    the next state on the low tiles (FP64 tiles copied through)."""
+    q, ny, nx = v.shape
+    types = np.ascontiguousarray(types, np.int32)
+    v = np.ascontiguousarray(v, np.float64)
+    words = np.ascontiguousarray(words, np.uint16)
+    off = np.ascontiguousarray(off, np.float64)
+    scl = np.ascontiguousarray(scl, np.float32)
+    wo, oo, so = words.copy(), off.copy(), scl.copy()
+    lib().oracle_gg_low_step(nx, ny, q, tile, _d(np.ascontiguousarray(p, np.float64)), stencil, bc, _i32(types),
+                             _d(v), _u16(words), _d(off), _f32(scl), _u16(wo), _d(oo), _f32(so))
+    return wo, oo, so

@@ -147,3 +147,63 @@ def test_sintering_low_eta_match_cpu_emulation(amps, O, bc, tile, cons):
                 # whose bounds came out one float apart shifts all its words)
                 assert np.mean(d[low] <= 0.01 * ulp[low] + doff[low] + 1e-300) > 0.95, (k, f)
         assert n_low > 0 and n_low < 3 * n * n
+
+
+@pytest.mark.parametrize("stencil,bc", [(0, 0), (1, 1)])
+def test_graingrowth_low_tiles_match_cpu_emulation(amps, O, stencil, bc):
+    """Adaptive grain growth (16x16 tiles, one type map per field, every low
+    field-tile fully updated): from the engine's raw state, one step of the
+    CPU emulation (oracle/low_emulation.cpp) must decode within one fp16 ulp
+    (times the tile's scale) plus the offsets' difference of the engine's
+    next state on every low field-tile, more than 90% of cells within 0.01
+    ulp, and the FP64 field-tiles must equal the FP64 oracle step of the
+    decoded state to 1e-12; three steps between monitor passes.  Reference
+    branches: graingrowth/function.h:74-228 (the type-1 half2 update)."""
+    from tests.helpers import smooth_fields
+    n, tile, q = 128, 16, 4
+    c = amps.Config.default(amps.GRAINGROWTH)
+    c.nx = c.ny = n
+    c.q, c.tile, c.stencil, c.bc = q, tile, stencil, bc
+    c.mode, c.monitor, c.monitor_field, c.threshold = amps.ADAPTIVE, amps.MON_GRADIENT, -1, 0.02
+    c.flags = amps.FLAG_NO_SKIP
+    p = np.array(c.params[:], np.float64)
+    a, b = smooth_fields(n, n, seed=9, nbump=3)
+    rng = np.random.default_rng(2)
+    init = [a, 1.0 - a, 0.2 * b, 1e-3 * rng.random((n, n))]
+    step0 = 25  # monitor passes at steps 20 and 30: none in between
+    with amps.Context(c) as ctx:
+        ctx.init()
+        for f, v in enumerate(init):
+            ctx.set_field(f, v)
+        ctx.step(step0)
+        ctx.sync()
+        n_low = 0
+        for k in range(3):
+            types = ctx.get_tile_types()
+            dec = np.stack([ctx.get_field(f) for f in range(q)])
+            raw = [ctx.get_raw_field(f) for f in range(q)]
+            ctx.step(1)
+            nxt = np.stack([ctx.get_field(f) for f in range(q)])
+            raw1 = [ctx.get_raw_field(f) for f in range(q)]
+            assert np.array_equal(ctx.get_tile_types(), types)
+            ew, eo, es = O.gg_low_step(types, dec, np.stack([r[0] for r in raw]), np.stack([r[1] for r in raw]),
+                                       np.stack([r[2] for r in raw]), p, tile, stencil, bc)
+            ref = O.gg_step(dec, p, stencil=stencil, bc=bc)
+            for f in range(q):
+                low = np.repeat(np.repeat(types[f] == 1, tile, 0), tile, 1)
+                n_low += int(low.sum())
+                gw, go, gs = raw1[f]
+                if (~low).any():
+                    assert rel_err(nxt[f][~low], ref[f][~low]) <= 1e-12, (k, f)
+                if not low.any():
+                    continue
+                g = O.decode_tiles(gw, go, gs, tile)
+                e = O.decode_tiles(ew[f], eo[f], es[f], tile)
+                assert np.array_equal(g[low], nxt[f][low])
+                ulp = np.maximum(O.f16_ulp_value(gw, gs, tile), O.f16_ulp_value(ew[f], es[f], tile))
+                doff = np.repeat(np.repeat(np.abs(go - eo[f]), tile, 0), tile, 1)
+                d = np.abs(g - e)
+                tol = 1.001 * ulp + doff + 1e-300
+                assert np.all(d[low] <= tol[low]), (k, f, float(np.max((d / tol)[low])))
+                assert np.mean(d[low] <= 0.01 * ulp[low] + doff[low] + 1e-300) > 0.9, (k, f)
+        assert 0 < n_low < 3 * q * n * n

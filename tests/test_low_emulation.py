@@ -169,3 +169,44 @@ def test_sintering_eta_encode_restatement(O):
     dev = np.abs(v - np.repeat(np.repeat(off, tile, 0), tile, 1))
     low = np.repeat(np.repeat(types == 1, tile, 0), tile, 1)
     assert np.all(np.abs(dec - v)[low] <= (0.5 * ulp + 2.0 ** -24 * dev + 1e-300)[low])
+
+
+@pytest.mark.parametrize("stencil,bc", [(0, 0), (1, 1)])
+def test_gg_all_low_step_close_to_fp64_step(O, stencil, bc):
+    """The grain-growth low-tile emulation on a state held entirely in low
+    tiles: one step equals the FP64 oracle step of the decoded state within
+    half an fp16 ulp plus the FP32 rounding of the deviations and of the
+    reaction terms, the update being far larger than that; with the coupling
+    2 gamma eta sum_{j!=i} eta_j^2 dropped (gamma = 0 in the emulator only)
+    the result moves by far more than the tolerance."""
+    n, tile, q = 64, 16, 3
+    p = O.gg_params_config()
+    a, b = smooth_fields(n, n, seed=5, nbump=4)
+    fields = [0.9 * a + 0.05, 0.8 * (1.0 - a) + 0.1 * b, 0.3 * b]
+    enc = [encode_grid(O, f, tile) for f in fields]
+    words = np.stack([e[0] for e in enc])
+    off = np.stack([e[1] for e in enc])
+    scl = np.stack([e[2] for e in enc])
+    dec = np.stack([O.decode_tiles(words[f], off[f], scl[f], tile) for f in range(q)])
+    types = np.ones((q, n // tile, n // tile), np.int32)
+    ew, eo, es = O.gg_low_step(types, dec, words, off, scl, p, tile, stencil, bc)
+    ref = O.gg_step(dec, p, stencil=stencil, bc=bc)
+    S = np.sum(dec * dec, axis=0)
+    inv = 1.0 / (p[O.GG["DX"]] * p[O.GG["DY"]])
+    dtl = p[O.GG["DT"]] * p[O.GG["L"]]
+    for f in range(q):
+        e = O.decode_tiles(ew[f], eo[f], es[f], tile)
+        ulp = O.f16_ulp_value(ew[f], es[f], tile)
+        eta = np.abs(dec[f])
+        react = 2 * p[O.GG["GAMMA"]] * eta * (S + eta * eta) + p[O.GG["BETA"]] * eta ** 3 + p[O.GG["ALPHA"]] * eta
+        s9 = box_scale(dec[f], off[f], tile)
+        tol = 0.5 * ulp + 32 * 2.0 ** -24 * (s9 * (1 + dtl * p[O.GG["KAPPA"]] * 12 * inv) + dtl * react + eta)
+        d = np.abs(e - ref[f])
+        assert np.all(d <= tol), (f, float(np.max(d / tol)))
+        assert np.max(np.abs(ref[f] - dec[f])) > 100 * np.median(tol)
+    p0 = p.copy()
+    p0[O.GG["GAMMA"]] = 0.0
+    nw, no, ns = O.gg_low_step(types, dec, words, off, scl, p0, tile, stencil, bc)
+    moved = max(np.max(np.abs(O.decode_tiles(nw[f], no[f], ns[f], tile) - ref[f]) / (0.5 * O.f16_ulp_value(ew[f], es[f], tile) + 1e-300))
+                for f in range(q))
+    assert moved > 20

@@ -71,6 +71,38 @@ __device__ inline T gg_lap(const T* B, int c, int LW, const T* cv, int i, bool m
     return (B[c - 1] + B[c + 1] + cv[i] + cv[i + 2] - T(4) * cv[i + 1]) * inv;
 }
 
+// Low-tile update of one cell, FP32 on the deviation dv from the tile's
+// offset (eta = off + dv), written with explicit fused multiply-adds so
+// that its rounding does not depend on the compiler's contraction (the
+// oracle's low-tile emulation restates it operation by operation):
+//   others = S - eta^2, df = 2 gamma eta others + (beta eta^2 - alpha) eta,
+//   r = dv - dt L (df - kappa lap)   (graingrowth/function.h:20-23).
+__device__ inline float gg_low_update(float dv, float lap, float offf, float Sf, float g2f, float bef, float alf,
+                                      float kapf, float dtlf) {
+    const float eta = offf + dv;
+    const float others = fmaf(-eta, eta, Sf);
+    const float t = fmaf(bef, eta * eta, -alf);
+    const float df = fmaf(g2f * eta, others, t * eta);
+    return fmaf(-dtlf, fmaf(-kapf, lap, df), dv);
+}
+
+// Low-tile Laplacians of the 4 cells of a row strip (FP32), with the centre
+// term fused: Moore as the 3x3 column sums minus 9 dv, 5-point as the four
+// neighbours minus 4 dv.
+__device__ inline void gg_lap_row4_low(const float (&u)[6], const float (&m)[6], const float (&d)[6], bool moore,
+                                       float inv, float (&lap)[4]) {
+    if (moore) {
+        float cs[6];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) cs[j] = (u[j] + m[j]) + d[j];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) lap[i] = fmaf(-9.0f, m[i + 1], (cs[i] + cs[i + 1]) + cs[i + 2]) * inv;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) lap[i] = fmaf(-4.0f, m[i + 1], ((m[i] + m[i + 2]) + u[i + 1]) + d[i + 1]) * inv;
+    }
+}
+
 // Halo element h (0..67) of a 16x16 tile: box position (by, bx) with
 // by, bx in [-1, 16], top row, bottom row, left column, right column.
 __device__ inline void gg_halo16(int h, int& by, int& bx) {
@@ -400,26 +432,11 @@ __global__ __launch_bounds__(64, 4) void gg_wave_kernel(StepArgs a, const InstDe
 #undef GG_ROW6F
                 }
                 float lapv[4], rn[4];
-                if (moore) {
-                    // column sums shared by the neighbouring cells (the low path's
-                    // own operand order: sum of the 3x3 block minus 9 eta)
-                    float cs[6];
-#pragma unroll
-                    for (int j = 0; j < 6; ++j) cs[j] = ru[j] + rm[j] + rd[j];
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) lapv[i] = (cs[i] + cs[i + 1] + cs[i + 2] - 9.0f * rm[i + 1]) * invf;
-                } else {
-                    gg_lap_row4<float>(ru, rm, rd, moore, invf, lapv);
-                }
+                gg_lap_row4_low(ru, rm, rd, moore, invf, lapv);
                 float mn = INFINITY, mx = -INFINITY;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const float dv = rm[i + 1];
-                    const float lap = lapv[i];
-                    const float eta = offf + dv;
-                    const float others = (float)S[i] - eta * eta;
-                    const float df = g2f * eta * others + bef * (eta * eta * eta) - alf * eta;
-                    rn[i] = dv - dtlf * (df - kapf * lap);
+                    rn[i] = gg_low_update(rm[i + 1], lapv[i], offf, (float)S[i], g2f, bef, alf, kapf, dtlf);
                     mn = fminf(mn, rn[i]);
                     mx = fmaxf(mx, rn[i]);
                 }
@@ -592,10 +609,7 @@ __global__ __launch_bounds__(256, 4) void gg_block_kernel(StepArgs a, const Inst
                     const int c = (ly0 + i + H) * LW + lx + H;
                     const float dv = cv[i + 1];
                     const float lap = gg_lap<float>(L.f, c, LW, cv, i, moore, invf);
-                    const float eta = offf + dv;
-                    const float others = (float)S[i] - eta * eta;
-                    const float df = g2f * eta * others + bef * (eta * eta * eta) - alf * eta;
-                    const float rn = dv - dtlf * (df - kapf * lap);
+                    const float rn = gg_low_update(dv, lap, offf, (float)S[i], g2f, bef, alf, kapf, dtlf);
                     r[i] = rn;
                     mn = fminf(mn, rn);
                     mx = fmaxf(mx, rn);
