@@ -191,7 +191,7 @@ def cpu_baseline(amps, cfg, seconds_budget):
     from oracle import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     p = np.array(cfg.params[:], dtype=np.float64)
-    # the same physics on 256^2, 2000 steps, in full
+    # the 256^2 x 2000-step workload (the same physics), in full
     p0 = p.copy()
     p0[amps.SF["CX"]] = p0[amps.SF["CY"]] = 128
     phi, T = O.sf_init(256, 256, p0)
