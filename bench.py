@@ -509,12 +509,12 @@ def main():
     fp64_tiles, low_tiles = amp["k_fp64_tiles"] / nl, amp["k_low_tiles"] / nl
     sweeps = {
         "sf_high_kernel<32,true,false>": (fp64_s, fp64_tiles * tile_cells * B_FP64, fp64_tiles),
-        "sf_lowfull_kernel<32,false>": (low_s, low_tiles * tile_cells * B_FP16, low_tiles),
+        "sf_low_tile_kernel<32,false>": (low_s, low_tiles * tile_cells * B_FP16, low_tiles),
     }
     dom = max(sweeps, key=lambda k: sweeps[k][0])
     t_dom, b_dom, n_dom = sweeps[dom]
     achieved = b_dom / t_dom / 1e9
-    tr = traffic_from_profiles("sf_lowfull_kernel" if "low" in dom else "sf_high_kernel_adaptive", workload_key,
+    tr = traffic_from_profiles("sf_low_tile_kernel" if "low" in dom else "sf_high_kernel_adaptive", workload_key,
                                amp["start_step"], amp["end_step"])
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": achieved / HBM_PEAK_GBS,

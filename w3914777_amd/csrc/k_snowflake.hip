@@ -1162,74 +1162,39 @@ __global__ __launch_bounds__(256, 3) void sf_mixed_kernel(StepArgs a, const Inst
     }
 }
 
-// Full low sweep of the NO_SKIP step: every tile of the low work-list runs
-// the full low update (sf_tile_low<TS, true>).  Persistent workgroups walk
-// the list at XCD-remapped positions with a two-deep software pipeline:
-// while tile k computes, the fp16 words, halo elements and codes of tile
-// k+1 are already in flight (their addresses need tile k+1's neighbour
-// types, which were loaded during tile k-1), so a tile's memory latency
-// overlaps the previous tile's arithmetic instead of following it.
+// Full low sweep of the NO_SKIP step: one workgroup per tile of the low
+// work-list (sf_tile_low<TS, true>).  The grid is the list's upper bound, the
+// tile count, and workgroups past the list's length exit at once.  With no
+// loop over tiles no per-tile constant stays live into the next tile (a
+// persistent loop with a software prefetch of the next tile kept them in
+// SGPRs spilled to VGPR lanes, needed 92 VGPRs against 48 here and ran 1.2x
+// slower), and the 6 resident workgroups of a CU (LDS-bound) hide each
+// other's load round.  Tile k's neighbour types are
+// loaded first (the halo addresses need them); the tile's own words, its
+// neighbours' codes and the halo then load in one round.
 template <int TS, bool MULTI>
-__global__ __launch_bounds__(256, 4) void sf_lowfull_kernel(StepArgs a, const InstDesc* __restrict__ inst,
-                                                            const int* __restrict__ tinst, const int* __restrict__ wl,
-                                                            const int* __restrict__ wlc) {
+__global__ __launch_bounds__(256) void sf_low_tile_kernel(StepArgs a, const InstDesc* __restrict__ inst,
+                                                          const int* __restrict__ tinst, const int* __restrict__ wl,
+                                                          const int* __restrict__ wlc) {
     __shared__ __align__(16) SfLowLds<TS> L;
-    const long long step = *a.step_ctr + a.step_add;
-    const int n = *wlc, G = gridDim.x;
-    int b = blockIdx.x;
-    if (b == 0 && threadIdx.x == 0) a.stats[3] += (unsigned long long)n;  // tiles updated (one writer)
+    const int n = *wlc, b = blockIdx.x;
+    if (b == 0 && threadIdx.x == 0) a.stats[3] += (unsigned long long)n;  // one writer
     if (b >= n) return;
-    // tile k = w (fields in f, codes in slot), tile k+1 = w1 (types t1)
-    int w = wl[xcd_remap(b, n)];
-    const bool more1 = b + G < n;
-    int w1 = more1 ? wl[xcd_remap(b + G, n)] : w;
-    // one instance: per-launch offsets of the interior tiles
+    const long long step = *a.step_ctr + a.step_add;
+    const int w = wl[xcd_remap(b, n)];
     SfTileConst<TS> tconst;
     if (!MULTI) sf_tile_const<TS>(tconst, inst[0]);
     const SfTileConst<TS>* tc = MULTI ? nullptr : &tconst;
-    unsigned hm;
+    const InstDesc& d = inst[MULTI ? tinst[w] : 0];
+    const unsigned hm = hmask_of(nb_type_lane<TS>(a, d, w, tc));
+    NbCode c;
+    nb_code_fetch<TS>(c, a, d, w, tc);
     LowFetch<TS> f;
-    {
-        const InstDesc& d = inst[MULTI ? tinst[w] : 0];
-        const int t = nb_type_lane<TS>(a, d, w, tc);
-        NbCode c;
-        nb_code_fetch<TS>(c, a, d, w, tc);
-        hm = hmask_of(t);
-        low_fetch<TS>(f, a, d, w, hm, tc);
-        nb_code_store<TS>(L, 0, c);
-    }
-    int t1 = nb_type_lane<TS>(a, inst[MULTI ? tinst[w1] : 0], w1, tc);
-    __syncthreads();  // slot 0 visible
-    int slot = 0;
-    for (; b < n; b += G) {
-        const bool more = b + G < n;
-        const int b2 = b + 2 * G;
-        const int w2 = b2 < n ? wl[xcd_remap(b2, n)] : w1;
-        // tile k+1: its fields and codes (issued before tile k computes),
-        // tile k+2: its neighbour types
-        const InstDesc& d1 = inst[MULTI ? tinst[w1] : 0];
-        const unsigned hm1 = hmask_of(t1);
-        LowFetch<TS> f1;
-        NbCode c1;
-        if (more) {
-            low_fetch<TS>(f1, a, d1, w1, hm1, tc);
-            nb_code_fetch<TS>(c1, a, d1, w1, tc);
-        } else {
-            f1 = f;
-            c1 = NbCode{0.0, 0.0, 0.0f, 0.0f};
-        }
-        const int t2 = b2 < n ? nb_type_lane<TS>(a, inst[MULTI ? tinst[w2] : 0], w2, tc) : 2;
-        const InstDesc& d = inst[MULTI ? tinst[w] : 0];
-        // stores c1 into slot ^ 1 after its first barrier, which its later
-        // barriers make visible (no barrier at the end: see sf_tile_low)
-        sf_tile_low<TS, true>(L, slot, c1, a, d, w, f, hm, step, tc);
-        slot ^= 1;
-        w = w1;
-        w1 = w2;
-        hm = hm1;
-        t1 = t2;
-        f = f1;
-    }
+    low_fetch<TS>(f, a, d, w, hm, tc);
+    nb_code_store<TS>(L, 0, c);
+    __syncthreads();  // neighbour table visible
+    // (the tile stores `c` again into slot 1, which nothing reads)
+    sf_tile_low<TS, true>(L, 0, c, a, d, w, f, hm, step, tc);
 }
 
 // ------------------------------------------------------------ host launch
@@ -1256,13 +1221,14 @@ template <int TS, bool MULTI>
 static void sf_launch(const StepArgs& a, bool adaptive, hipStream_t s, const hipEvent_t* ev) {
     dim3 blk(WG);
     if (adaptive && (a.flags & AMPS_FLAG_NO_SKIP)) {
-        // the FP64 tiles, then every low tile (no hold sweep)
+        // the FP64 tiles (persistent: the one-tile form measured no faster
+        // on them), then every low tile, one workgroup each (no hold sweep)
         constexpr auto kh = sf_high_kernel<TS, true, MULTI>;
-        constexpr auto kl = sf_lowfull_kernel<TS, MULTI>;
+        constexpr auto kl = sf_low_tile_kernel<TS, MULTI>;
         launch_sweep(kh, dim3(resident_grid<kh>(a.total_tiles, a.grid_cap)), blk, s, ev, a, a.inst, a.tile_inst,
                      a.worklist, a.wl_count);
-        launch_sweep(kl, dim3(resident_grid<kl>(a.total_tiles, a.grid_cap)), blk, s, ev ? ev + 2 : nullptr, a, a.inst,
-                     a.tile_inst, a.wl_low, a.wl_low_count);
+        launch_sweep(kl, dim3(a.total_tiles), blk, s, ev ? ev + 2 : nullptr, a, a.inst, a.tile_inst, a.wl_low,
+                     a.wl_low_count);
     } else if (adaptive) {
         // bulk-hold sweep first (it lists the low tiles it does not hold),
         // then FP64 tiles and those low tiles in one launch
