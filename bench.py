@@ -410,8 +410,10 @@ def main():
     ap.add_argument("--profile-run", action="store_true",
                     help="only one path's trajectory and timed windows (for rocprofv3 runs); prints the tile "
                          "updates of the run")
-    ap.add_argument("--profile-path", choices=("adaptive", "fp64", "sintering", "graingrowth"), default="adaptive",
-                    help="with --profile-run: the path to run (sintering: 300 steps of the 1024^2 sintering "
+    ap.add_argument("--profile-path", choices=("adaptive", "half2", "fp64", "sintering", "graingrowth"),
+                    default="adaptive",
+                    help="with --profile-run: the path to run (half2: the adaptive path with "
+                         "AMPS_FLAG_HALF2; sintering: 300 steps of the 1024^2 sintering "
                          "workload, FP64 then adaptive NO_SKIP; graingrowth: the Q=36 2048^2 workload's "
                          "10000 steps, FP64 on 32^2 tiles then adaptive NO_SKIP at --gg-threshold on 16^2)")
     args = ap.parse_args()
@@ -460,10 +462,14 @@ def main():
     if args.profile_run:
         if args.copy_probe:
             copy_bandwidth(reps=1)
-        r = timed_run(amps, amp_cfg if args.profile_path == "adaptive" else f64_cfg, horizon, K, R, W, fields=False)
+        pc = {"adaptive": amp_cfg, "half2": amp_cfg.copy(), "fp64": f64_cfg}[args.profile_path]
+        if args.profile_path == "half2":
+            pc.flags = amps.FLAG_NO_SKIP | amps.FLAG_HALF2
+        r = timed_run(amps, pc, horizon, K, R, W, fields=False)
         i = r["info"]
         n = i.steps_done
-        print(json.dumps({"profile_run": workload_key if args.profile_path == "adaptive" else "fp64", "steps": n,
+        key = {"adaptive": workload_key, "half2": workload_key + "_half2", "fp64": "fp64"}[args.profile_path]
+        print(json.dumps({"profile_run": key, "steps": n,
                           "fp64_tile_updates": i.fp64_tile_updates, "low_tile_updates": i.low_full_tile_updates}))
         return
 
@@ -559,6 +565,29 @@ def main():
         "note": "not the headline: low tiles in the bulk phase advance T only (phi within 1e-8 of 0 or 1), "
                 "T-flat tiles are held, the low-tile noise term is skipped where |phi(1-phi)| <= 1e-6; "
                 "the FP64 path has no such skipping"}
+    # the packed-half2 form of the low tiles (AMPS_FLAG_HALF2), at every
+    # dilation of the frontier
+    h2_rows = []
+    for d in sorted({args.dilate} | ({int(x) for x in args.frontier.split(",")} if args.frontier else set())):
+        c = amp_cfg.copy()
+        c.flags, c.dilate = amps.FLAG_NO_SKIP | amps.FLAG_HALF2, d
+        r = timed_run(amps, c, horizon, K, R, W, kernel_pass=d == args.dilate)
+        v = rate(cells, K, r)
+        e = field_errors(r["fields"], f64["fields"])
+        row = {"dilate": d, "value": v, "speedup_amp_over_fp64": v / f64_value,
+               "speedup_over_fp32_low_tiles": v / value if d == args.dilate else None,
+               "phi_l2_rms": e["phi"]["l2_rms"], "T_l2_rms": e["T"]["l2_rms"], "phi_linf": e["phi"]["linf"]}
+        if d == args.dilate:
+            nl = max(r["k_launches"], 1)
+            row["sf_low_tile_kernel_h2_us"] = r["k_low_ms"] / nl * 1e3
+            row["sf_high_kernel_us"] = r["k_fp64_ms"] / nl * 1e3
+        h2_rows.append(row)
+        log(f"[bench] half2 d={d}: {v / f64_value:.2f}x, phi rms {row['phi_l2_rms']:.3g}")
+    out["half2_variant"] = {
+        "flags": "NO_SKIP|HALF2", "rows": h2_rows,
+        "note": "low tiles with the six-fold anisotropy and the driving force's atan in packed half2 "
+                "(v_pk_fma_f16), gradients, Laplacians, flux sums and the update in FP32; not the headline, "
+                "which keeps the FP32 arithmetic"}
     if not args.no_others:
         out["other_workloads"] = other_workloads(amps, args)
     if not args.no_cpu:

@@ -105,8 +105,18 @@ enum {
  *             c' = c + dt div(M grad mu) with face mobilities
  *             M_{i+1/2} = (M_i + M_{i+1})/2, in FP64 (mass-conserving up to
  *             the [0,1] clamp).  Without it, the reference's c' = c + dt M lap mu
- *             (sintering/function.h:95-101). */
-enum { AMPS_FLAG_NO_SKIP = 1, AMPS_FLAG_CH_CONSERVATIVE = 2 };
+ *             (sintering/function.h:95-101).
+ * HALF2     : snowflake low tiles evaluate their pointwise terms in packed
+ *             half2 (v_pk_fma_f16) on the offset-and-scaled fields: the
+ *             six-fold unit vector (cos 6 theta, sin 6 theta) of each ring
+ *             cell and its rotation by 6 theta0 (two ring cells per packed
+ *             operation), and the atan of the driving force (two cells per
+ *             operation, a degree-7 odd polynomial on min(|x|, 1/|x|)).
+ *             Gradients, Laplacians, flux sums, eps, eps' and the update
+ *             stay in FP32.  Needs NO_SKIP and tile 32; an aniso other than 6
+ *             keeps the FP32 anisotropy.  The reference's low branch is half2
+ *             throughout (snowflake/function.h:417-537). */
+enum { AMPS_FLAG_NO_SKIP = 1, AMPS_FLAG_CH_CONSERVATIVE = 2, AMPS_FLAG_HALF2 = 4 };
 
 /* Field indices: snowflake 0 = phi, 1 = T; grain growth 0..q-1 = eta_i;
  * sintering 0 = c, 1 = eta_1, 2 = eta_2. */

@@ -22,6 +22,10 @@
 //             hw < 2^e (e >= -100; hw = 0 gives scl = 0), h = fp16_rne((r -
 //             mid) 2^(14-e)), new offset = off + mid (double)
 //
+// With half2 set (the engine's AMPS_FLAG_HALF2 snowflake tiles) the six-fold
+// unit vector with its rotation and the driving force's atan follow the
+// engine's packed-half2 sequence instead, one fp16 rounding per operation.
+//
 // The engine fuses multiply-adds and uses hardware reciprocal and atan
 // approximations of ~1 ulp, so results agree to within rounding of the FP32
 // arithmetic: decoded values within one fp16 quantum (scl) of the tile.
@@ -158,6 +162,59 @@ void unit_power(float gx, float gy, int n, float& c, float& s) {
     s = wi / pw;
 }
 
+// ---- packed-half2 terms (the engine's AMPS_FLAG_HALF2 low tiles).  Every
+// fp16 operation rounds once (to nearest even, subnormals kept, overflow to
+// infinity).  Values are carried as doubles that are fp16 values: a product
+// of two is exact in double, and the sums and fused multiply-adds below span
+// fewer than 53 bits, so rounding the double result to fp16 is the one
+// rounding of the fp16 operation.
+double rh(double x) {
+    if (x == 0.0 || !std::isfinite(x)) return x;
+    const double a = std::fabs(x);
+    int e;
+    std::frexp(a, &e);                       // a = m 2^e, m in [0.5, 1)
+    const int q = e - 11 < -24 ? -24 : e - 11;  // quantum: 11 significant bits, subnormals 2^-24
+    double r = std::ldexp(std::nearbyint(std::ldexp(a, -q)), q);  // RNE (default mode)
+    if (r > 65504.0) r = INFINITY;
+    return std::copysign(r, x);
+}
+
+// (cos 6(theta - theta0), sin 6(theta - theta0)) from the rescaled gradient
+// (u, v), as the engine's packed sequence: w = (u + iv)^2 = A + iB,
+// (A (A^2 - 3B^2), B (3A^2 - B^2)) / (R2 R2 R2) with R2 = u^2 + v^2, then the
+// rotation by the fp16 cos/sin of 6 theta0.
+void aniso6_h2(float u, float v, float cth0, float sth0, float& ca, float& sa) {
+    const double U = rh(u), V = rh(v);
+    const double UU = rh(U * U), VV = rh(V * V);
+    const double R2 = rh(UU + VV), A = rh(UU - VV);
+    const double B = rh(rh(U + U) * V);
+    const double AA = rh(A * A), BB = rh(B * B);
+    const double Cn = rh(A * rh(std::fma(BB, -3.0, AA)));
+    const double Sn = rh(B * rh(std::fma(AA, 3.0, -BB)));
+    const double inv = rh(1.0 / rh(rh(R2 * R2) * R2));
+    const double C = rh(Cn * inv), S = rh(Sn * inv);
+    const double c0 = rh(cth0), s0 = rh(sth0);
+    ca = (float)rh(std::fma(S, s0, rh(C * c0)));
+    sa = (float)rh(std::fma(S, c0, -rh(C * s0)));
+}
+
+// atan in fp16: z = min(|x|, 1/|x|), p = z (c0 + c1 z^2 + c2 z^4 + c3 z^6),
+// pi/2 - p where |x| > 1 (as (pi/2 - 2p) f + p with f = clamp(2048 |x| - 2048,
+// 0, 1)), the sign of x.  x is the FP32 argument.
+float atan_h2(float x) {
+    const double c0 = rh(0.99926763f), c1 = rh(-0.32142954f), c2 = rh(0.14661292f), c3 = rh(-0.03913253f);
+    const double X = rh(x), AX = std::fabs(X);
+    const double Z = std::fmin(AX, rh(1.0 / AX));
+    const double Z2 = rh(Z * Z);
+    double p = rh(std::fma(Z2, c3, c2));
+    p = rh(std::fma(Z2, p, c1));
+    p = rh(std::fma(Z2, p, c0));
+    p = rh(p * Z);
+    const double f = std::fmin(std::fmax(rh(std::fma(AX, 2048.0, -2048.0)), 0.0), 1.0);
+    const double r = rh(std::fma(f, rh(std::fma(p, -2.0, rh(1.5707963705062866))), p));
+    return std::copysign((float)r, std::signbit(X) ? -1.0f : 1.0f);
+}
+
 }  // namespace
 
 extern "C" {
@@ -174,7 +231,7 @@ extern "C" {
 void oracle_sf_low_step(int nx, int ny, int tile, const double* p, int stencil, int bc, uint64_t seed, int64_t step,
                         const int32_t* types, const double* phi, const double* T, const uint16_t* w,
                         const double* off, const float* scl, float noise_skip, uint16_t* w_out, double* off_out,
-                        float* scl_out) {
+                        float* scl_out, int half2) {
     const int TS = tile, H = 2, LW = TS + 2 * H, RW = TS + 2;
     const int tx = nx / TS, ty = ny / TS, nt = tx * ty;
     const size_t ncell = (size_t)nx * ny;
@@ -233,9 +290,19 @@ void oracle_sf_low_step(int nx, int ny, int tile, const double* p, int stencil, 
                 const int c = (sy + 1) * LW + sx + 1;
                 const float gx = (bp[c + 1] - bp[c - 1]) * inv2dx;
                 const float gy = (bp[c + LW] - bp[c - LW]) * inv2dy;
-                float cn, sn;
-                unit_power(gx, gy, n, cn, sn);
-                const float ca = cn * cth0 + sn * sth0, sa = sn * cth0 - cn * sth0;
+                float ca, sa;
+                if (half2 && n == 6) {
+                    const float mx = std::fmax(std::fabs(gx), std::fabs(gy));
+                    int ex;
+                    std::frexp(mx, &ex);
+                    const float u = mx > 0.0f ? std::ldexp(gx, -ex) : 1.0f, v = mx > 0.0f ? std::ldexp(gy, -ex) : 0.0f;
+                    aniso6_h2(u, v, cth0, sth0, ca, sa);
+                } else {
+                    float cn, sn;
+                    unit_power(gx, gy, n, cn, sn);
+                    ca = cn * cth0 + sn * sth0;
+                    sa = sn * cth0 - cn * sth0;
+                }
                 const float eps = epsb + epsbd * ca, epsd = mend * sa;
                 const float A = eps * epsd;
                 P[ry * RW + rx] = A * gx;
@@ -261,7 +328,7 @@ void oracle_sf_low_step(int nx, int ny, int tile, const double* p, int stencil, 
                 const float flux = term1 + term2 + E2[r] * lp;
                 const float ph = bp[b], tt = bt[b];
                 const float pa = phi_off + ph;
-                const float m = aop * std::atan(gam * (teq_m - tt));
+                const float m = aop * (half2 ? atan_h2(gam * (teq_m - tt)) : std::atan(gam * (teq_m - tt)));
                 const float pp = pa * (1.0f - pa);
                 float br = flux + pp * (pa - 0.5f + m);
                 if (noise != 0.0f && std::fabs(pp) > noise_skip) {
@@ -383,6 +450,13 @@ void oracle_gg_low_step(int nx, int ny, int q, int tile, const double* p, int st
 
 // fp16 word -> double, for the tests' decoding of emulated tiles.
 double oracle_h2d(uint16_t h) { return (double)h2f(h); }
+double oracle_rh(double x) { return rh(x); }
+void oracle_atan_h2(const float* x, float* r, int n) {
+    for (int i = 0; i < n; ++i) r[i] = atan_h2(x[i]);
+}
+void oracle_aniso6_h2(const float* u, const float* v, float cth0, float sth0, float* ca, float* sa, int n) {
+    for (int i = 0; i < n; ++i) aniso6_h2(u[i], v[i], cth0, sth0, ca[i], sa[i]);
+}
 uint16_t oracle_f2h(float x) { return f2h(x); }
 
 }  // extern "C"

@@ -81,7 +81,11 @@ def lib():
         U16P = ctypes.POINTER(ctypes.c_uint16)
         FP = ctypes.POINTER(ctypes.c_float)
         L.oracle_sf_low_step.argtypes = [I, I, I, D, I, I, U64, I64, I32P, D, D, U16P, D, FP, ctypes.c_float,
-                                         U16P, D, FP]
+                                         U16P, D, FP, I]
+        L.oracle_rh.argtypes = [ctypes.c_double]
+        L.oracle_rh.restype = ctypes.c_double
+        L.oracle_atan_h2.argtypes = [FP, FP, I]
+        L.oracle_aniso6_h2.argtypes = [FP, FP, ctypes.c_float, ctypes.c_float, FP, FP, I]
         L.oracle_h2d.argtypes = [ctypes.c_uint16]
         L.oracle_h2d.restype = ctypes.c_double
         L.oracle_f2h.argtypes = [ctypes.c_float]
@@ -326,9 +330,10 @@ def _f32(a):
 
 
 def sf_low_step(types, phi, T, words, off, scl, p, tile, stencil=STENCIL_REF, bc=BC_PERIODIC, seed=1, step=0,
-                noise_skip=0.0):
+                noise_skip=0.0, half2=False):
     """One step of the low tiles (type 1) of a snowflake grid, emulating the
-    engine's fp16-storage / FP32-arithmetic scheme (low_emulation.cpp).
+    engine's fp16-storage / FP32-arithmetic scheme (low_emulation.cpp);
+    half2: the AMPS_FLAG_HALF2 form (packed-half2 anisotropy and atan).
     words [2, ny, nx] uint16, off [2, ty, tx] float64, scl [2, ty, tx]
     float32; phi, T hold the FP64 tiles' values.  Returns (words, off, scl)."""
     ny, nx = phi.shape
@@ -339,8 +344,32 @@ def sf_low_step(types, phi, T, words, off, scl, p, tile, stencil=STENCIL_REF, bc
     wo, oo, so = np.empty_like(words), np.empty_like(off), np.empty_like(scl)
     lib().oracle_sf_low_step(nx, ny, tile, _d(p), stencil, bc, seed, step, _i32(types),
                              _d(np.ascontiguousarray(phi, np.float64)), _d(np.ascontiguousarray(T, np.float64)),
-                             _u16(words), _d(off), _f32(scl), float(noise_skip), _u16(wo), _d(oo), _f32(so))
+                             _u16(words), _d(off), _f32(scl), float(noise_skip), _u16(wo), _d(oo), _f32(so),
+                             int(bool(half2)))
     return wo, oo, so
+
+
+def fp16_round(x):
+    """The emulator's rounding of a double to fp16 (nearest even), as a double."""
+    return lib().oracle_rh(float(x))
+
+
+def atan_h2(x):
+    """The engine's packed-half2 atan (AMPS_FLAG_HALF2) of float32 arguments."""
+    x = np.ascontiguousarray(x, np.float32)
+    r = np.empty_like(x)
+    lib().oracle_atan_h2(_f32(x), _f32(r), x.size)
+    return r
+
+
+def aniso6_h2(u, v, cth0, sth0):
+    """The engine's packed-half2 (cos 6(theta-theta0), sin 6(theta-theta0)) of
+    rescaled gradients (u, v) (max(|u|,|v|) in [0.5, 1), or (1, 0))."""
+    u = np.ascontiguousarray(u, np.float32)
+    v = np.ascontiguousarray(v, np.float32)
+    ca, sa = np.empty_like(u), np.empty_like(u)
+    lib().oracle_aniso6_h2(_f32(u), _f32(v), float(cth0), float(sth0), _f32(ca), _f32(sa), u.size)
+    return ca, sa
 
 
 _H2D = None

@@ -101,6 +101,9 @@ def test_paras_expressions(amps, tmp_path):
     (lambda c: c.params.__setitem__(2, 1e-2), -1),       # dt above the explicit-Euler bound
     (lambda c: c.params.__setitem__(0, float("nan")), -1),
     (lambda c: (setattr(c, "mode", 2), setattr(c, "monitor", 0)), -1),
+    (lambda c: setattr(c, "flags", 8), -1),                                   # unknown flags bit
+    (lambda c: (setattr(c, "tile", 16), setattr(c, "flags", 5)), -7),         # HALF2 needs tile 32
+    (lambda c: (setattr(c, "tile", 32), setattr(c, "flags", 4)), -7),         # HALF2 needs NO_SKIP
 ])
 def test_create_rejects_invalid_config_before_device_work(amps, mutate, status):
     c = amps.Config.default(amps.SNOWFLAKE)

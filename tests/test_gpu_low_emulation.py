@@ -9,7 +9,9 @@ and the
 FP64 tiles must match the FP64 oracle step of the decoded state to 1e-12.
 Steps are chosen between monitor passes, so the tile types do not change.
 The reference branch this stands in for is snowflake/function.h:417-537
-(the type-1 half2 update), whose own arithmetic cannot be run here.
+(the type-1 half2 update), whose own arithmetic cannot be run here; the
+HALF2 cases run the engine's packed-half2 anisotropy and atan against the
+emulator's fp16 restatement of them.
 
 Sintering's low tiles (the etas in fp16; c and every update in FP64) are
 checked the same way against the FP64 oracle step of the decoded state and
@@ -23,11 +25,13 @@ from tests.helpers import rel_err
 pytestmark = pytest.mark.gpu
 
 CASES = [
-    # mode, n, tile, stencil, bc, dilate, monitor threshold
-    ("adaptive", 256, 32, 1, 1, 1, 0.05),
-    ("adaptive", 256, 16, 0, 1, 0, 0.05),
-    ("low", 128, 16, 0, 0, 0, 0.0),
-    ("low", 128, 32, 1, 0, 0, 0.0),
+    # mode, n, tile, stencil, bc, dilate, monitor threshold, HALF2
+    ("adaptive", 256, 32, 1, 1, 1, 0.05, False),
+    ("adaptive", 256, 16, 0, 1, 0, 0.05, False),
+    ("low", 128, 16, 0, 0, 0, 0.0, False),
+    ("low", 128, 32, 1, 0, 0, 0.0, False),
+    ("adaptive", 256, 32, 1, 1, 1, 0.05, True),
+    ("low", 128, 32, 0, 0, 0, 0.0, True),
 ]
 
 
@@ -37,8 +41,8 @@ def raw_state(ctx):
     return np.stack([w0, w1]), np.stack([o0, o1]), np.stack([s0, s1])
 
 
-@pytest.mark.parametrize("mode,n,tile,stencil,bc,dil,thr", CASES)
-def test_low_tiles_match_cpu_emulation(amps, O, mode, n, tile, stencil, bc, dil, thr):
+@pytest.mark.parametrize("mode,n,tile,stencil,bc,dil,thr,h2", CASES)
+def test_low_tiles_match_cpu_emulation(amps, O, mode, n, tile, stencil, bc, dil, thr, h2):
     c = amps.Config.default(amps.SNOWFLAKE)
     c.nx = c.ny = n
     c.tile, c.stencil, c.bc = tile, stencil, bc
@@ -47,7 +51,7 @@ def test_low_tiles_match_cpu_emulation(amps, O, mode, n, tile, stencil, bc, dil,
         c.params[i] = p[i]
     c.params[O.SF["R2"]] = p[O.SF["R2"]] = 60.0
     c.seed = 17
-    c.flags, c.dilate = amps.FLAG_NO_SKIP, dil
+    c.flags, c.dilate = amps.FLAG_NO_SKIP | (amps.FLAG_HALF2 if h2 else 0), dil
     if mode == "adaptive":
         c.mode, c.monitor, c.threshold = amps.ADAPTIVE, amps.MON_GRADIENT, thr
     else:
@@ -66,7 +70,8 @@ def test_low_tiles_match_cpu_emulation(amps, O, mode, n, tile, stencil, bc, dil,
             phi1, T1 = ctx.get_field(0), ctx.get_field(1)
             w1, off1, scl1 = raw_state(ctx)
             assert np.array_equal(ctx.get_tile_types(), types)  # no monitor pass in between
-            ew, eo, es = O.sf_low_step(types, phi, T, w, off, scl, p, tile, stencil, bc, seed=c.seed, step=step0 + k)
+            ew, eo, es = O.sf_low_step(types, phi, T, w, off, scl, p, tile, stencil, bc, seed=c.seed, step=step0 + k,
+                                       half2=h2)
             low = np.repeat(np.repeat(types == 1, tile, 0), tile, 1)
             n_low += int(low.sum())
             for f, (got_f, (gw, go, gs), (xw, xo, xs)) in enumerate(

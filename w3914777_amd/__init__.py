@@ -25,7 +25,7 @@ SNOWFLAKE, GRAINGROWTH, SINTERING = 1, 2, 3
 PURE_FP64, PURE_HALF2, ADAPTIVE, STATIC_MAP = 0, 1, 2, 3
 MON_NONE, MON_SPATIAL, MON_TEMPORAL, MON_GRADIENT = 0, 1, 2, 3
 SWEEP_ALL, SWEEP_FP64, SWEEP_LOW = 0, 1, 2
-FLAG_NO_SKIP, FLAG_CH_CONSERVATIVE = 1, 2
+FLAG_NO_SKIP, FLAG_CH_CONSERVATIVE, FLAG_HALF2 = 1, 2, 4
 STENCIL_REF, STENCIL_LAP5 = 0, 1
 BC_PERIODIC, BC_NEUMANN = 0, 1
 INIT_REF, INIT_CONFIG = 0, 1

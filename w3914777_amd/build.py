@@ -57,7 +57,23 @@ def build(verbose=False):
         raise RuntimeError("link failed")
     os.replace(tmp, LIB)
     build_driver()
+    build_probes()
     return LIB
+
+
+def build_probes():
+    """The VALU issue-cost probe (probes/valu_rates.hip), a measurement tool
+    beside the library."""
+    src = os.path.join(HERE, "probes", "valu_rates.hip")
+    out = os.path.join(HERE, "probes", "libvalu_rates.so")
+    tmp = out + ".tmp"
+    cmd = [HIPCC, "-O3", "-std=c++17", "-fPIC", "-shared"] + DEVICE + [src, "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("probe build failed")
+    os.replace(tmp, out)
+    return out
 
 
 def build_driver():

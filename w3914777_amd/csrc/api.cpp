@@ -157,7 +157,11 @@ int validate(const amps_config& c) {
     if (c.init != AMPS_INIT_REF && c.init != AMPS_INIT_CONFIG) return fail(AMPS_E_ARG, "init out of range");
     if (!finite_params(c)) return fail(AMPS_E_ARG, "non-finite parameter");
     if (c.wg_per_cu < 0 || c.wg_per_cu > 64) return fail(AMPS_E_ARG, "wg_per_cu must be in [0, 64]");
-    if (c.flags & ~(AMPS_FLAG_NO_SKIP | AMPS_FLAG_CH_CONSERVATIVE)) return fail(AMPS_E_ARG, "unknown flags bit");
+    if (c.flags & ~(AMPS_FLAG_NO_SKIP | AMPS_FLAG_CH_CONSERVATIVE | AMPS_FLAG_HALF2))
+        return fail(AMPS_E_ARG, "unknown flags bit");
+    if ((c.flags & AMPS_FLAG_HALF2) &&
+        (c.model != AMPS_SNOWFLAKE || c.tile != 32 || !(c.flags & AMPS_FLAG_NO_SKIP)))
+        return fail(AMPS_E_UNSUPPORTED, "HALF2 applies to the snowflake with tile 32 and NO_SKIP");
     if ((c.flags & AMPS_FLAG_CH_CONSERVATIVE) && c.model != AMPS_SINTERING)
         return fail(AMPS_E_ARG, "CH_CONSERVATIVE applies to the sintering model only");
     if (c.dilate < 0 || c.dilate > 16) return fail(AMPS_E_ARG, "dilate must be in [0, 16]");

@@ -172,12 +172,135 @@ __device__ inline void sf_lap(const Rt* F, int ly, int x0, bool ref, Rt inv_dxdy
     }
 }
 
+// ------------------------------------------------------------ packed half2
+// Pointwise terms of the low tiles in packed half2 (AMPS_FLAG_HALF2): each
+// operation below is one v_pk_* instruction on two cells, rounded once to
+// fp16 (round to nearest even); `fp contract(off)` keeps every written
+// product and sum a rounding of its own, and the fused multiply-adds are the
+// explicit __builtin_elementwise_fma calls.  Inputs arrive in FP32 and
+// results leave in FP32.
+typedef _Float16 sf_h2 __attribute__((ext_vector_type(2)));
+
+__device__ inline sf_h2 h2_of(float a, float b) { return sf_h2{(_Float16)a, (_Float16)b}; }
+__device__ inline sf_h2 h2_splat(float a) { return sf_h2{(_Float16)a, (_Float16)a}; }
+__device__ inline sf_h2 h2_fma(sf_h2 a, sf_h2 b, sf_h2 c) { return __builtin_elementwise_fma(a, b, c); }
+
+// (cos n(theta - theta0), sin n(theta - theta0)) for n = 6 of two ring cells
+// from their exactly rescaled gradients (u, v) (max(|u|, |v|) in [0.5, 1), or
+// u = 1, v = 0 for a zero gradient): w = (u + i v)^2 = A + i B, the unit
+// vector w^3 / |w|^3 = (A (A^2 - 3 B^2), B (3 A^2 - B^2)) / R^3 with R = |w|,
+// then the rotation by 6 theta0 (host cos/sin).
+__device__ inline void aniso6_h2(float u0, float v0, float u1, float v1, float cth0, float sth0, float& ca0,
+                                 float& sa0, float& ca1, float& sa1) {
+#pragma clang fp contract(off)
+    const sf_h2 U = h2_of(u0, u1), V = h2_of(v0, v1);
+    const sf_h2 UU = U * U, VV = V * V;
+    const sf_h2 R2 = UU + VV, A = UU - VV;
+    const sf_h2 B = (U + U) * V;
+    const sf_h2 AA = A * A, BB = B * B;
+    const sf_h2 Cn = A * h2_fma(BB, h2_splat(-3.0f), AA);
+    const sf_h2 Sn = B * h2_fma(AA, h2_splat(3.0f), -BB);
+    const sf_h2 inv = h2_splat(1.0f) / ((R2 * R2) * R2);
+    const sf_h2 C = Cn * inv, S = Sn * inv;
+    const sf_h2 c0 = h2_splat(cth0), s0 = h2_splat(sth0);
+    const sf_h2 CA = h2_fma(S, s0, C * c0);
+    const sf_h2 SA = h2_fma(S, c0, -(C * s0));
+    ca0 = (float)CA.x; ca1 = (float)CA.y;
+    sa0 = (float)SA.x; sa1 = (float)SA.y;
+}
+
+// atan of two arguments: z = min(|x|, 1/|x|), p = z (c0 + c1 z^2 + c2 z^4 +
+// c3 z^6) (least-squares fit of atan on [0, 1], 8.7e-5 in exact arithmetic;
+// 1.5e-3 absolute with the fp16 roundings over |x| <= 20), pi/2 - p where
+// |x| > 1 (f = clamp(2048 (|x| - 1), 0, 1) selects it: |x| > 1 in fp16 means
+// |x| - 1 >= 2^-10), and the sign of x.
+constexpr float SF_ATAN_H2_C0 = 0.99926763f, SF_ATAN_H2_C1 = -0.32142954f, SF_ATAN_H2_C2 = 0.14661292f,
+                SF_ATAN_H2_C3 = -0.03913253f;
+
+__device__ inline void atan_h2(float x0, float x1, float& r0, float& r1) {
+#pragma clang fp contract(off)
+    const sf_h2 X = h2_of(x0, x1);
+    // |X| by its bits: the reciprocal then reads these fp16 values (an
+    // elementwise abs lets the compiler re-round the FP32 product into it)
+    const sf_h2 AX = __builtin_bit_cast(sf_h2, __builtin_bit_cast(unsigned, X) & 0x7fff7fffu);
+    const sf_h2 Z = __builtin_elementwise_min(AX, h2_splat(1.0f) / AX);
+    const sf_h2 Z2 = Z * Z;
+    sf_h2 p = h2_fma(Z2, h2_splat(SF_ATAN_H2_C3), h2_splat(SF_ATAN_H2_C2));
+    p = h2_fma(Z2, p, h2_splat(SF_ATAN_H2_C1));
+    p = h2_fma(Z2, p, h2_splat(SF_ATAN_H2_C0));
+    p = p * Z;
+    const sf_h2 f = __builtin_elementwise_min(
+        __builtin_elementwise_max(h2_fma(AX, h2_splat(2048.0f), h2_splat(-2048.0f)), h2_splat(0.0f)), h2_splat(1.0f));
+    const sf_h2 r = h2_fma(f, h2_fma(p, h2_splat(-2.0f), h2_splat(1.5707963705062866f)), p);
+    // r >= 0: the sign bits of x complete it
+    const sf_h2 rs = __builtin_bit_cast(
+        sf_h2, __builtin_bit_cast(unsigned, r) | (__builtin_bit_cast(unsigned, X) & 0x80008000u));
+    r0 = (float)rs.x;
+    r1 = (float)rs.y;
+}
+
+// Phase 1 of the low tiles with the six-fold unit vector and its rotation in
+// packed half2: thread t pairs ring cells e and e + RW^2/2 (e = t + k WG), so
+// that a wave's LDS reads and writes stay on consecutive words for both
+// (pairing neighbouring cells of a row strides them by two words: 2-way bank
+// conflicts); the gradients, the exact power-of-two rescale, eps, eps' and
+// P, Q, E2 stay in FP32 as in sf_phase1.  An aniso other than 6 runs
+// sf_phase1.
+template <int TS>
+__device__ inline void sf_phase1_h2(const float* phi, float* P, float* Q, float* E2, const InstDesc& d, int ox, int oy,
+                                    float inv2dx, float inv2dy, float cth0, float sth0, float epsb, float epsbd,
+                                    float mend) {
+    using G = SfGeo<TS>;
+    static_assert((G::RW * G::RW) % 2 == 0, "the ring splits into two halves");
+    constexpr int HALF = G::RW * G::RW / 2;
+    static_assert(G::LW == G::RW + 2 && G::PW == G::RW + 2, "ring and box rows differ by 2");
+    if ((int)d.k[SFK_ANISO_N] != 6) {
+        sf_phase1<TS, float>(phi, P, Q, E2, d, ox, oy, inv2dx, inv2dy, cth0, sth0, epsb, epsbd, mend, false);
+        return;
+    }
+    const bool edge = d.bc != AMPS_BC_PERIODIC &&
+                      (ox == 0 || oy == 0 || ox + TS == d.nx || oy + TS == d.ny);
+    for (int e = threadIdx.x; e < HALF; e += WG) {
+        int cb[2], ob[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int r = e + k * HALF;
+            const int ry0 = r / G::RW, rx0 = r - ry0 * G::RW;
+            // box index (ry0 + 1) LW + rx0 + 1 = r + 2 ry0 + LW + 1, ring index
+            // RB + ry0 PW + rx0 - 1 = r + 2 ry0 + RB - 1
+            cb[k] = edge ? (ring_src(ry0, oy, d.ny, d.bc) + 1) * G::LW + ring_src(rx0, ox, d.nx, d.bc) + 1
+                         : r + 2 * ry0 + G::LW + 1;
+            ob[k] = r + 2 * ry0 + G::RB - 1;
+        }
+        float gx[2], gy[2], u[2], v[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            gx[k] = (phi[cb[k] + 1] - phi[cb[k] - 1]) * inv2dx;
+            gy[k] = (phi[cb[k] + G::LW] - phi[cb[k] - G::LW]) * inv2dy;
+            const float mx = fmaxf(fabsf(gx[k]), fabsf(gy[k]));
+            const int ex = frexp_exp(mx);
+            u[k] = mx > 0.0f ? ldexp_any(gx[k], -ex) : 1.0f;  // zero gradient: theta = 0
+            v[k] = ldexp_any(gy[k], -ex);
+        }
+        float ca[2], sa[2];
+        aniso6_h2(u[0], v[0], u[1], v[1], cth0, sth0, ca[0], sa[0], ca[1], sa[1]);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const float eps = epsb + epsbd * ca[k], epsd = mend * sa[k];
+            const float A = eps * epsd;
+            P[ob[k]] = A * gx[k];
+            Q[ob[k]] = A * gy[k];
+            E2[ob[k]] = eps * eps;
+        }
+    }
+}
+
 // Phase 2 for the R cells (row ly, columns x0 .. x0+R-1) of this thread:
 // writes the new values to np/nt (and the old ones to pc/tc).  The noise
 // term a*phi(1-phi)(u-1/2) is evaluated only where |phi(1-phi)| > noise_skip:
 // 0 on the FP64 path (bitwise the same result), a bound far below the fp16
 // resolution on the low path.
-template <int TS, typename Rt>
+template <int TS, typename Rt, bool H2 = false>
 __device__ inline void sf_phase2(const Rt* F, const Rt* G, const Rt* P, const Rt* Q, const Rt* E2, int ly, int x0,
                                  Rt* pc, Rt* tc, Rt* np, Rt* nt, bool ref, Rt inv_dxdy, Rt inv2dx,
                                  Rt inv2dy, Rt dtt, Rt dt, Rt kap, Rt aop, Rt gam, Rt teq_minus_off, Rt phi_off,
@@ -203,11 +326,25 @@ __device__ inline void sf_phase2(const Rt* F, const Rt* G, const Rt* P, const Rt
             flux[i] = term1 + term2 + e2[i] * lp[i];
         }
     }
+    Rt mv[R];
+    if constexpr (H2) {
+        static_assert(R % 2 == 0, "half2 driving force pairs the cells of a thread");
+#pragma unroll
+        for (int i = 0; i < R; i += 2) {
+            float r0, r1;
+            atan_h2(gam * (teq_minus_off - tc[i]), gam * (teq_minus_off - tc[i + 1]), r0, r1);
+            mv[i] = aop * r0;
+            mv[i + 1] = aop * r1;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < R; ++i) mv[i] = aop * atan(gam * (teq_minus_off - tc[i]));
+    }
 #pragma unroll
     for (int i = 0; i < R; ++i) {
         const Rt ph = pc[i], t = tc[i];
         const Rt pa = phi_off + ph;
-        const Rt m = aop * atan(gam * (teq_minus_off - t));
+        const Rt m = mv[i];
         const Rt pp = pa * (Rt(1) - pa);
         Rt br = flux[i] + pp * (pa - Rt(0.5) + m);
         if (noise != Rt(0) && fabs(pp) > noise_skip) {
@@ -554,7 +691,7 @@ __device__ inline void nb_code_store(SfLowLds<TS>& L, int slot, const NbCode& c)
 // its last read of this tile's values.
 // NOSKIP: no bulk hold and no noise skip (every term of the update is
 // evaluated, as the reference's type-1 branch does).
-template <int TS, bool NOSKIP>
+template <int TS, bool NOSKIP, bool H2 = false>
 __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const StepArgs& a, const InstDesc& d,
                             int gtile, const LowFetch<TS>& f, unsigned hmask, long long step,
                             const SfTileConst<TS>* tcon = nullptr) {
@@ -650,13 +787,18 @@ __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const 
         return;
     }
 
-    sf_phase1<TS, float>(L.phi, L.P, L.Q, L.E2, d, ox, oy, d.kf[SFK_INV_2DX], d.kf[SFK_INV_2DY],
+    if constexpr (H2)
+        sf_phase1_h2<TS>(L.phi, L.P, L.Q, L.E2, d, ox, oy, d.kf[SFK_INV_2DX], d.kf[SFK_INV_2DY],
                          d.kf[SFK_COS_N_THETA0], d.kf[SFK_SIN_N_THETA0], d.kf[SFK_EPSB], d.kf[SFK_EPSB_DELTA],
-                         d.kf[SFK_MEPSB_ANISO_DELTA], tcon != nullptr);
+                         d.kf[SFK_MEPSB_ANISO_DELTA]);
+    else
+        sf_phase1<TS, float>(L.phi, L.P, L.Q, L.E2, d, ox, oy, d.kf[SFK_INV_2DX], d.kf[SFK_INV_2DY],
+                             d.kf[SFK_COS_N_THETA0], d.kf[SFK_SIN_N_THETA0], d.kf[SFK_EPSB], d.kf[SFK_EPSB_DELTA],
+                             d.kf[SFK_MEPSB_ANISO_DELTA], tcon != nullptr);
     __syncthreads();  // B
 
     float rp[R], rt[R];
-    sf_phase2<TS, float>(L.phi, L.T, L.P, L.Q, L.E2, ly, x0, pc, tc, rp, rt,
+    sf_phase2<TS, float, H2>(L.phi, L.T, L.P, L.Q, L.E2, ly, x0, pc, tc, rp, rt,
                          d.stencil == AMPS_STENCIL_REF, d.kf[SFK_INV_DXDY], d.kf[SFK_INV_2DX], d.kf[SFK_INV_2DY],
                          d.kf[SFK_DT_OVER_TAU], d.kf[SFK_DT], d.kf[SFK_KAPPA], d.kf[SFK_ALPHA_OVER_PI],
                          d.kf[SFK_GAMMA], (float)(d.k[SFK_TEQ] - offT), (float)offp, d.kf[SFK_NOISE],
@@ -1172,7 +1314,7 @@ __global__ __launch_bounds__(256, 3) void sf_mixed_kernel(StepArgs a, const Inst
 // other's load round.  Tile k's neighbour types are
 // loaded first (the halo addresses need them); the tile's own words, its
 // neighbours' codes and the halo then load in one round.
-template <int TS, bool MULTI>
+template <int TS, bool MULTI, bool H2>
 __global__ __launch_bounds__(256) void sf_low_tile_kernel(StepArgs a, const InstDesc* __restrict__ inst,
                                                           const int* __restrict__ tinst, const int* __restrict__ wl,
                                                           const int* __restrict__ wlc) {
@@ -1194,7 +1336,7 @@ __global__ __launch_bounds__(256) void sf_low_tile_kernel(StepArgs a, const Inst
     nb_code_store<TS>(L, 0, c);
     __syncthreads();  // neighbour table visible
     // (the tile stores `c` again into slot 1, which nothing reads)
-    sf_tile_low<TS, true>(L, 0, c, a, d, w, f, hm, step, tc);
+    sf_tile_low<TS, true, H2>(L, 0, c, a, d, w, f, hm, step, tc);
 }
 
 // ------------------------------------------------------------ host launch
@@ -1224,11 +1366,18 @@ static void sf_launch(const StepArgs& a, bool adaptive, hipStream_t s, const hip
         // the FP64 tiles (persistent: the one-tile form measured no faster
         // on them), then every low tile, one workgroup each (no hold sweep)
         constexpr auto kh = sf_high_kernel<TS, true, MULTI>;
-        constexpr auto kl = sf_low_tile_kernel<TS, MULTI>;
         launch_sweep(kh, dim3(resident_grid<kh>(a.total_tiles, a.grid_cap)), blk, s, ev, a, a.inst, a.tile_inst,
                      a.worklist, a.wl_count);
-        launch_sweep(kl, dim3(a.total_tiles), blk, s, ev ? ev + 2 : nullptr, a, a.inst, a.tile_inst, a.wl_low,
-                     a.wl_low_count);
+        // HALF2 is validated to tile 32 (R = 4 cells per thread, in pairs)
+        if constexpr (TS == 32) {
+            if (a.flags & AMPS_FLAG_HALF2) {
+                launch_sweep(sf_low_tile_kernel<TS, MULTI, true>, dim3(a.total_tiles), blk, s, ev ? ev + 2 : nullptr,
+                             a, a.inst, a.tile_inst, a.wl_low, a.wl_low_count);
+                return;
+            }
+        }
+        launch_sweep(sf_low_tile_kernel<TS, MULTI, false>, dim3(a.total_tiles), blk, s, ev ? ev + 2 : nullptr, a,
+                         a.inst, a.tile_inst, a.wl_low, a.wl_low_count);
     } else if (adaptive) {
         // bulk-hold sweep first (it lists the low tiles it does not hold),
         // then FP64 tiles and those low tiles in one launch
