@@ -514,8 +514,8 @@ def main():
     fp64_s, low_s = amp["k_fp64_ms"] / nl / 1e3, amp["k_low_ms"] / nl / 1e3
     fp64_tiles, low_tiles = amp["k_fp64_tiles"] / nl, amp["k_low_tiles"] / nl
     sweeps = {
-        "sf_high_kernel<32,true,false>": (fp64_s, fp64_tiles * tile_cells * B_FP64, fp64_tiles),
-        "sf_low_tile_kernel<32,false>": (low_s, low_tiles * tile_cells * B_FP16, low_tiles),
+        "sf_high_kernel<32,true,false,512>": (fp64_s, fp64_tiles * tile_cells * B_FP64, fp64_tiles),
+        "sf_low_tile_kernel<32,false,false>": (low_s, low_tiles * tile_cells * B_FP16, low_tiles),
     }
     dom = max(sweeps, key=lambda k: sweeps[k][0])
     t_dom, b_dom, n_dom = sweeps[dom]
@@ -533,7 +533,7 @@ def main():
     out["adaptive_step_kernels_us"]["tiles_per_step"] = {k: v[2] for k, v in sweeps.items()}
     kf_s = f64["k_fp64_ms"] / max(f64["k_launches"], 1) / 1e3
     fa = B_FP64 * cells / kf_s / 1e9
-    out["fp64"]["roofline"] = {"kernel": "sf_f64_tile_kernel<32,false>", "avg_launch_us": kf_s * 1e6,
+    out["fp64"]["roofline"] = {"kernel": "sf_f64_tile_kernel<32,false,512>", "avg_launch_us": kf_s * 1e6,
                                "achieved": fa, "peak": HBM_PEAK_GBS, "frac": fa / HBM_PEAK_GBS,
                                "frac_of_device_copy": fa / copy_gbs}
 

@@ -34,12 +34,12 @@
 
 namespace amps {
 
-// Geometry of a snowflake tile of TS x TS cells.
-template <int TS>
+// Geometry of a snowflake tile of TS x TS cells updated by NT threads.
+template <int TS, int NT = WG>
 struct SfGeo {
     static constexpr int H = 2;
     static constexpr int LW = TS + 2 * H;          // box row stride (16-byte multiple for float and double)
-    static constexpr int R = TS * TS / WG;         // cells per thread
+    static constexpr int R = TS * TS / NT;         // cells per thread
     static constexpr int TPR = TS / R;             // threads per row
     static constexpr int RW = TS + 2;              // ring box rows/cols
     // Ring arrays: ring cell (ry, rx) at RB + ry*PW + rx - 1, so that the ring
@@ -49,7 +49,7 @@ struct SfGeo {
     static constexpr int RB = 4;
     static constexpr int RSZ = RB + RW * PW;
     static constexpr int NH = LW * LW - TS * TS;   // halo frame elements
-    static constexpr int NHT = (NH + WG - 1) / WG;
+    static constexpr int NHT = (NH + NT - 1) / NT;
 };
 
 template <int TS, typename Rt>
@@ -92,7 +92,7 @@ __device__ inline int ring_src(int r, int o, int n, int bc) {
 //
 // fast: index the ring directly where no ring cell needs the zero-flux
 // mirror (a tile off the domain edge, or periodic boundaries).
-template <int TS, typename Rt>
+template <int TS, typename Rt, int NT = WG>
 __device__ inline void sf_phase1(const Rt* phi, Rt* P, Rt* Q, Rt* E2, const InstDesc& d, int ox, int oy,
                                  Rt inv2dx, Rt inv2dy, Rt cth0, Rt sth0, Rt epsb, Rt epsbd, Rt mend,
                                  bool fast = false) {
@@ -104,7 +104,7 @@ __device__ inline void sf_phase1(const Rt* phi, Rt* P, Rt* Q, Rt* E2, const Inst
         // box index (ry0 + 1) LW + rx0 + 1 = e + 2 ry0 + LW + 1, ring index
         // RB + ry0 PW + rx0 - 1 = e + 2 ry0 + RB - 1 (LW = PW = RW + 2)
         static_assert(G::LW == G::RW + 2 && G::PW == G::RW + 2, "ring and box rows differ by 2");
-        for (int e = threadIdx.x; e < G::RW * G::RW; e += WG) {
+        for (int e = threadIdx.x; e < G::RW * G::RW; e += NT) {
             const int ry0 = e / G::RW;
             const int c = e + 2 * ry0 + G::LW + 1, o = e + 2 * ry0 + G::RB - 1;
             Rt gx = (phi[c + 1] - phi[c - 1]) * inv2dx;
@@ -118,7 +118,7 @@ __device__ inline void sf_phase1(const Rt* phi, Rt* P, Rt* Q, Rt* E2, const Inst
         }
         return;
     }
-    for (int e = threadIdx.x; e < G::RW * G::RW; e += WG) {
+    for (int e = threadIdx.x; e < G::RW * G::RW; e += NT) {
         const int ry0 = e / G::RW, rx0 = e - ry0 * G::RW;
         int ry = ry0, rx = rx0;
         if (edge) {
@@ -140,9 +140,9 @@ __device__ inline void sf_phase1(const Rt* phi, Rt* P, Rt* Q, Rt* E2, const Inst
 
 // Laplacian of the R cells (row ly, columns x0..x0+R-1) of a box field;
 // also returns the cells' own values.  Reads are 16-byte aligned rows.
-template <int TS, typename Rt>
+template <int TS, typename Rt, int NT = WG>
 __device__ inline void sf_lap(const Rt* F, int ly, int x0, bool ref, Rt inv_dxdy, Rt* own, Rt* lap) {
-    using Gm = SfGeo<TS>;
+    using Gm = SfGeo<TS, NT>;
     constexpr int H = Gm::H, LW = Gm::LW, R = Gm::R;
     const int b = (ly + H) * LW + x0;
     Rt row[R + 4];
@@ -300,16 +300,16 @@ __device__ inline void sf_phase1_h2(const float* phi, float* P, float* Q, float*
 // term a*phi(1-phi)(u-1/2) is evaluated only where |phi(1-phi)| > noise_skip:
 // 0 on the FP64 path (bitwise the same result), a bound far below the fp16
 // resolution on the low path.
-template <int TS, typename Rt, bool H2 = false>
+template <int TS, typename Rt, bool H2 = false, int NT = WG>
 __device__ inline void sf_phase2(const Rt* F, const Rt* G, const Rt* P, const Rt* Q, const Rt* E2, int ly, int x0,
                                  Rt* pc, Rt* tc, Rt* np, Rt* nt, bool ref, Rt inv_dxdy, Rt inv2dx,
                                  Rt inv2dy, Rt dtt, Rt dt, Rt kap, Rt aop, Rt gam, Rt teq_minus_off, Rt phi_off,
                                  Rt noise, Rt noise_skip, uint32_t key, int gcell0) {
-    using Gm = SfGeo<TS>;
+    using Gm = SfGeo<TS, NT>;
     constexpr int R = Gm::R, PW = Gm::PW, RB = Gm::RB;
     Rt lp[R], lt[R];
-    sf_lap<TS, Rt>(F, ly, x0, ref, inv_dxdy, pc, lp);
-    sf_lap<TS, Rt>(G, ly, x0, ref, inv_dxdy, tc, lt);
+    sf_lap<TS, Rt, NT>(F, ly, x0, ref, inv_dxdy, pc, lp);
+    sf_lap<TS, Rt, NT>(G, ly, x0, ref, inv_dxdy, tc, lt);
     // ring arrays: cell (ly, x) is ring (ly+1, x+1), stored at RB + (ly+1)*PW + x
     const int rb = RB + (ly + 1) * PW + x0;
     Rt flux[R];
@@ -360,21 +360,21 @@ __device__ inline void sf_phase2(const Rt* F, const Rt* G, const Rt* P, const Rt
 // ------------------------------------------------------------- FP64 tile
 // Halo frame of the tile (box coordinates and cell offsets) staged by this
 // thread: element h = threadIdx.x + k*WG of the frame.
-template <int TS>
+template <int TS, int NT = WG>
 struct SfHalo {
-    int off[SfGeo<TS>::NHT];
-    int box[SfGeo<TS>::NHT];
-    int reg[SfGeo<TS>::NHT];
+    int off[SfGeo<TS, NT>::NHT];
+    int box[SfGeo<TS, NT>::NHT];
+    int reg[SfGeo<TS, NT>::NHT];
 };
 
-template <int TS>
-__device__ inline void sf_halo_map(SfHalo<TS>& hm, const InstDesc& d, int ox, int oy) {
-    using G = SfGeo<TS>;
+template <int TS, int NT = WG>
+__device__ inline void sf_halo_map(SfHalo<TS, NT>& hm, const InstDesc& d, int ox, int oy) {
+    using G = SfGeo<TS, NT>;
     constexpr int H = G::H;
     const bool interior = ox >= H && oy >= H && ox + TS + H <= d.nx && oy + TS + H <= d.ny;
 #pragma unroll
     for (int k = 0; k < G::NHT; ++k) {
-        const int h = threadIdx.x + k * WG;
+        const int h = threadIdx.x + k * NT;
         int bly = 0, blx = 0;
         if (h < G::NH) halo_coord<TS>(h, bly, blx);
         int gy = oy - H + bly, gx = ox - H + blx;
@@ -446,15 +446,15 @@ __device__ inline void sf_halo_of(SfHalo<TS>& hm, const InstDesc& d, int tile_x,
     }
 }
 
-template <int TS, bool ADAPT>
-__device__ inline void sf_halo_load(uint2* hp, uint2* ht, const StepArgs& a, const InstDesc& d, const SfHalo<TS>& hm,
-                                    unsigned hmask) {
+template <int TS, bool ADAPT, int NT = WG>
+__device__ inline void sf_halo_load(uint2* hp, uint2* ht, const StepArgs& a, const InstDesc& d,
+                                    const SfHalo<TS, NT>& hm, unsigned hmask) {
     const double* fphi = a.in.f64 + d.cell_base;
     const double* fT = a.in.f64 + a.total_cells + d.cell_base;
     const unsigned short* hphi = reinterpret_cast<const unsigned short*>(a.in.f16 + d.cell_base);
     const unsigned short* hT = reinterpret_cast<const unsigned short*>(a.in.f16 + a.total_cells + d.cell_base);
 #pragma unroll
-    for (int k = 0; k < SfGeo<TS>::NHT; ++k) {
+    for (int k = 0; k < SfGeo<TS, NT>::NHT; ++k) {
         const int off = hm.off[k];
         if (off < 0) continue;
         const bool h64 = !ADAPT || ((hmask >> hm.reg[k]) & 1u);
@@ -463,10 +463,10 @@ __device__ inline void sf_halo_load(uint2* hp, uint2* ht, const StepArgs& a, con
     }
 }
 
-template <int TS, bool ADAPT>
+template <int TS, bool ADAPT, int NT = WG>
 __device__ void sf_tile_f64(SfLds<TS, double>& L, const StepArgs& a, const InstDesc& d, int tile_x, int tile_y,
                             unsigned hmask, long long step) {
-    using G = SfGeo<TS>;
+    using G = SfGeo<TS, NT>;
     constexpr int H = G::H, LW = G::LW, R = G::R;
     const int ox = tile_x * TS, oy = tile_y * TS;
     const int ly = threadIdx.x / G::TPR, x0 = (threadIdx.x % G::TPR) * R;
@@ -487,12 +487,12 @@ __device__ void sf_tile_f64(SfLds<TS, double>& L, const StepArgs& a, const InstD
 #pragma unroll
         for (int j = 0; j < R; ++j) { pc[j] = fphi[gcell0 + j]; tc[j] = fT[gcell0 + j]; }
     }
-    SfHalo<TS> hm;
-    sf_halo_map<TS>(hm, d, ox, oy);
+    SfHalo<TS, NT> hm;
+    sf_halo_map<TS, NT>(hm, d, ox, oy);
     // halo elements as raw words, one load sequence for both storages (see
     // sf_halo_load): FP64 bits or the aligned 4-word fp16 group
     uint2 hp[G::NHT], ht[G::NHT];
-    sf_halo_load<TS, ADAPT>(hp, ht, a, d, hm, hmask);
+    sf_halo_load<TS, ADAPT, NT>(hp, ht, a, d, hm, hmask);
     if (ADAPT) {
         Nb nb;
         nb_setup(nb, d, tile_x, tile_y, TS);
@@ -521,13 +521,13 @@ __device__ void sf_tile_f64(SfLds<TS, double>& L, const StepArgs& a, const InstD
     }
     __syncthreads();
 
-    sf_phase1<TS, double>(L.phi, L.P, L.Q, L.E2, d, ox, oy, d.k[SFK_INV_2DX], d.k[SFK_INV_2DY],
+    sf_phase1<TS, double, NT>(L.phi, L.P, L.Q, L.E2, d, ox, oy, d.k[SFK_INV_2DX], d.k[SFK_INV_2DY],
                           d.k[SFK_COS_N_THETA0], d.k[SFK_SIN_N_THETA0], d.k[SFK_EPSB], d.k[SFK_EPSB_DELTA],
                           d.k[SFK_MEPSB_ANISO_DELTA]);
     __syncthreads();
 
     double np[R], nt[R];
-    sf_phase2<TS, double>(L.phi, L.T, L.P, L.Q, L.E2, ly, x0, pc, tc, np, nt,
+    sf_phase2<TS, double, false, NT>(L.phi, L.T, L.P, L.Q, L.E2, ly, x0, pc, tc, np, nt,
                           d.stencil == AMPS_STENCIL_REF, d.k[SFK_INV_DXDY], d.k[SFK_INV_2DX], d.k[SFK_INV_2DY],
                           d.k[SFK_DT_OVER_TAU], d.k[SFK_DT], d.k[SFK_KAPPA], d.k[SFK_ALPHA_OVER_PI], d.k[SFK_GAMMA],
                           d.k[SFK_TEQ], 0.0, d.k[SFK_NOISE], 0.0, noise_key(d.seed, (uint64_t)step), gcell0);
@@ -1171,8 +1171,8 @@ __device__ inline unsigned hmask_of(int t) {
     return (unsigned)(__ballot((threadIdx.x & 63) < 9 && t == 2) & 0x1FFull);
 }
 
-template <int TS, bool ADAPT, bool MULTI>
-__global__ __launch_bounds__(256, 3) void sf_high_kernel(StepArgs a, const InstDesc* __restrict__ inst,
+template <int TS, bool ADAPT, bool MULTI, int NT = WG>
+__global__ __launch_bounds__(NT, 3) void sf_high_kernel(StepArgs a, const InstDesc* __restrict__ inst,
                                                          const int* __restrict__ tinst, const int* __restrict__ wl,
                                                          const int* __restrict__ wlc) {
     __shared__ __align__(16) SfLds<TS, double> L;
@@ -1191,7 +1191,7 @@ __global__ __launch_bounds__(256, 3) void sf_high_kernel(StepArgs a, const InstD
         const InstDesc& d = inst[MULTI ? tinst[w] : 0];
         const int lt = w - d.tile_base;
         const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
-        sf_tile_f64<TS, ADAPT>(L, a, d, tile_x, tile_y, ADAPT ? hmask_of(tcur) : 0x1FFu, step);
+        sf_tile_f64<TS, ADAPT, NT>(L, a, d, tile_x, tile_y, ADAPT ? hmask_of(tcur) : 0x1FFu, step);
         __syncthreads();
         w = wn;
         tcur = tnext;
@@ -1199,13 +1199,14 @@ __global__ __launch_bounds__(256, 3) void sf_high_kernel(StepArgs a, const InstD
 }
 
 // PURE_FP64 sweep: one workgroup per tile (the grid is the work count).
-// With no loop over tiles nothing stays live from one tile to the next, so
-// the tile fits in 114 VGPRs (a persistent loop keeps the per-tile constants
-// live and needs 138, spilling SGPRs); occupancy is then bound by the LDS
-// boxes at 3 workgroups per CU either way.
-template <int TS, bool MULTI>
-__global__ __launch_bounds__(256) void sf_f64_tile_kernel(StepArgs a, const InstDesc* __restrict__ inst,
-                                                          const int* __restrict__ tinst) {
+// With no loop over tiles nothing stays live from one tile to the next (a
+// persistent loop keeps the per-tile constants live, spilling SGPRs).  The
+// LDS boxes (50.5 KB) allow 3 workgroups per CU; 32^2 tiles run with NT =
+// 512 threads, 2 cells each (78 VGPRs against 114 at 256 threads and 4
+// cells), so 24 waves per CU share the tile chains' latency instead of 12.
+template <int TS, bool MULTI, int NT = WG>
+__global__ __launch_bounds__(NT) void sf_f64_tile_kernel(StepArgs a, const InstDesc* __restrict__ inst,
+                                                         const int* __restrict__ tinst) {
     __shared__ __align__(16) SfLds<TS, double> L;
     const long long step = *a.step_ctr + a.step_add;
     const int b = blockIdx.x;
@@ -1214,7 +1215,7 @@ __global__ __launch_bounds__(256) void sf_f64_tile_kernel(StepArgs a, const Inst
     const InstDesc& d = inst[MULTI ? tinst[w] : 0];
     const int lt = w - d.tile_base;
     const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
-    sf_tile_f64<TS, false>(L, a, d, tile_x, tile_y, 0x1FFu, step);
+    sf_tile_f64<TS, false, NT>(L, a, d, tile_x, tile_y, 0x1FFu, step);
 }
 
 // Mixed sweep of the adaptive step: the FP64 tiles (FP64 work-list) and the
@@ -1365,9 +1366,11 @@ static void sf_launch(const StepArgs& a, bool adaptive, hipStream_t s, const hip
     if (adaptive && (a.flags & AMPS_FLAG_NO_SKIP)) {
         // the FP64 tiles (persistent: the one-tile form measured no faster
         // on them), then every low tile, one workgroup each (no hold sweep)
-        constexpr auto kh = sf_high_kernel<TS, true, MULTI>;
-        launch_sweep(kh, dim3(resident_grid<kh>(a.total_tiles, a.grid_cap)), blk, s, ev, a, a.inst, a.tile_inst,
-                     a.worklist, a.wl_count);
+        // 32^2 tiles: 512 threads (2 cells each), as in the PURE_FP64 sweep
+        constexpr int NTH = TS == 32 ? 512 : WG;
+        constexpr auto kh = sf_high_kernel<TS, true, MULTI, NTH>;
+        launch_sweep(kh, dim3(resident_grid<kh, NTH>(a.total_tiles, a.grid_cap)), dim3(NTH), s, ev, a, a.inst,
+                     a.tile_inst, a.worklist, a.wl_count);
         // HALF2 is validated to tile 32 (R = 4 cells per thread, in pairs)
         if constexpr (TS == 32) {
             if (a.flags & AMPS_FLAG_HALF2) {
@@ -1394,8 +1397,9 @@ static void sf_launch(const StepArgs& a, bool adaptive, hipStream_t s, const hip
         launch_sweep(km, dim3(resident_grid<km>(a.total_tiles, a.grid_cap)), blk, s, ev ? ev + 2 : nullptr, a, a.inst,
                      a.tile_inst, a.worklist, a.wl_count, (const int*)a.full_seg, (const int*)a.full_cnt);
     } else {
-        constexpr auto kh = sf_f64_tile_kernel<TS, MULTI>;
-        launch_sweep(kh, dim3(a.nwork), blk, s, ev, a, a.inst, a.tile_inst);
+        // 32^2 tiles: 512 threads, 2 cells each (see sf_f64_tile_kernel)
+        constexpr int NT = TS == 32 ? 512 : WG;
+        launch_sweep(sf_f64_tile_kernel<TS, MULTI, NT>, dim3(a.nwork), dim3(NT), s, ev, a, a.inst, a.tile_inst);
     }
 }
 
