@@ -481,15 +481,15 @@ struct GgBlockLds {
     int ntype[AMPS_MAX_FIELDS][9];
     double noff[AMPS_MAX_FIELDS][9];
     float nscl[AMPS_MAX_FIELDS][9];
-    float red[8];
+    float red[16];  // block_minmax of up to 8 waves
 };
 
 // Raw box values of field f: FP64 bits, or the fp16 word where the element's
 // tile is low (the type table in LDS says which storage is current).
-template <int TS, bool ADAPT>
+template <int TS, bool ADAPT, int NT = WG>
 __device__ inline void gg_load_raw(unsigned long long* raw, const StepArgs& a, const InstDesc& d, int f,
-                                   const BoxMap<TS, 1>& m, const GgBlockLds<TS>& L) {
-    constexpr int N = BoxMap<TS, 1>::N;
+                                   const BoxMap<TS, 1, NT>& m, const GgBlockLds<TS>& L) {
+    constexpr int N = BoxMap<TS, 1, NT>::N;
     const double* f64 = a.in.f64 + (size_t)f * a.total_cells + d.cell_base;
     const unsigned short* f16 =
         reinterpret_cast<const unsigned short*>(a.in.f16 + (size_t)f * a.total_cells + d.cell_base);
@@ -506,11 +506,14 @@ __device__ inline void gg_load_raw(unsigned long long* raw, const StepArgs& a, c
 // tile needs more registers than a SIMD can give 4 waves).  Per tile: one memory round brings the neighbour precision table of every
 // field, the carried sum of squares and field 0's box; afterwards the box of
 // field f+1 is loaded into registers while field f is computed, so each field
-// costs one overlapped load round.
-template <int TS, bool ADAPT, bool MULTI, bool ONE>
-__global__ __launch_bounds__(256, 4) void gg_block_kernel(StepArgs a, const InstDesc* __restrict__ inst,
+// costs one overlapped load round.  NT threads own column strips of
+// TS*TS/NT cells; the FP64 sweep runs NT = 512 (2 cells per thread, 54
+// VGPRs against 82 at 256 and 4 cells): twice the waves per CU share each
+// tile's load rounds.
+template <int TS, bool ADAPT, bool MULTI, bool ONE, int NT = WG>
+__global__ __launch_bounds__(NT, 4) void gg_block_kernel(StepArgs a, const InstDesc* __restrict__ inst,
                                                       const int* __restrict__ tinst, const int* __restrict__ types) {
-    constexpr int H = 1, LW = TS + 2, R = TS * TS / WG, N = BoxMap<TS, 1>::N;
+    constexpr int H = 1, LW = TS + 2, R = TS * TS / NT, N = BoxMap<TS, 1, NT>::N;
     __shared__ GgBlockLds<TS> L;
     for (int b = blockIdx.x, it = 0; b < a.nwork && (!ONE || it == 0); b += gridDim.x, ++it) {
         const int w = xcd_remap(b, a.nwork);
@@ -518,8 +521,8 @@ __global__ __launch_bounds__(256, 4) void gg_block_kernel(StepArgs a, const Inst
         const int lt = w - d.tile_base;
         const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
         const int ox = tile_x * TS, oy = tile_y * TS;
-        BoxMap<TS, H> m;
-        box_map<TS, H>(m, d, tile_x, tile_y);
+        BoxMap<TS, H, NT> m;
+        box_map<TS, H, NT>(m, d, tile_x, tile_y);
         const double inv = d.k[GGK_INV_DXDY], dtl = d.k[GGK_DT_L], kap = d.k[GGK_KAPPA];
         const double al = d.k[GGK_ALPHA], be = d.k[GGK_BETA], g2 = d.k[GGK_2GAMMA];
         const bool moore = d.stencil == AMPS_STENCIL_REF;
@@ -528,7 +531,7 @@ __global__ __launch_bounds__(256, 4) void gg_block_kernel(StepArgs a, const Inst
         if (ADAPT) {
             Nb nb;
             nb_setup(nb, d, tile_x, tile_y, TS);
-            for (int e = threadIdx.x; e < a.q * 9; e += WG) {
+            for (int e = threadIdx.x; e < a.q * 9; e += NT) {
                 const int f = e / 9, k = e - f * 9;
                 const int g = nb_tile(nb, k);
                 L.ntype[f][k] = types[(size_t)f * a.total_tiles + g];
@@ -546,7 +549,7 @@ __global__ __launch_bounds__(256, 4) void gg_block_kernel(StepArgs a, const Inst
             Sn[i] = 0.0;
         }
         unsigned long long raw[N];
-        gg_load_raw<TS, ADAPT>(raw, a, d, 0, m, L);
+        gg_load_raw<TS, ADAPT, NT>(raw, a, d, 0, m, L);
         for (int f = 0; f < a.q; ++f) {
             const int ty = ADAPT ? L.ntype[f][4] : 2;
             // stage field f from the raw values, then start loading field f+1
@@ -555,7 +558,7 @@ __global__ __launch_bounds__(256, 4) void gg_block_kernel(StepArgs a, const Inst
                 for (int k = 0; k < N; ++k) {
                     if (m.off[k] < 0) continue;
                     const int r = m.reg[k];
-                    L.d[threadIdx.x + k * WG] =
+                    L.d[threadIdx.x + k * NT] =
                         (!ADAPT || L.ntype[f][r] == 2)
                             ? __longlong_as_double((long long)raw[k])
                             : L.noff[f][r] + (double)__half2float(__ushort_as_half((unsigned short)raw[k])) *
@@ -573,10 +576,10 @@ __global__ __launch_bounds__(256, 4) void gg_block_kernel(StepArgs a, const Inst
                     else
                         x = (float)(L.noff[f][r] - own_off) +
                             __half2float(__ushort_as_half((unsigned short)raw[k])) * L.nscl[f][r];
-                    L.f[threadIdx.x + k * WG] = x;
+                    L.f[threadIdx.x + k * NT] = x;
                 }
             }
-            if (f + 1 < a.q) gg_load_raw<TS, ADAPT>(raw, a, d, f + 1, m, L);
+            if (f + 1 < a.q) gg_load_raw<TS, ADAPT, NT>(raw, a, d, f + 1, m, L);
             __syncthreads();
             if (ty == 2) {
                 double* out = a.out.f64 + (size_t)f * a.total_cells + d.cell_base;
@@ -614,7 +617,7 @@ __global__ __launch_bounds__(256, 4) void gg_block_kernel(StepArgs a, const Inst
                     mn = fminf(mn, rn);
                     mx = fmaxf(mx, rn);
                 }
-                block_minmax(mn, mx, L.red);
+                block_minmax<NT / 64>(mn, mx, L.red);
                 const TileCode tc = tile_code(mn, mx);
                 const double noff = own_off + (double)tc.mid;
                 __half* out = a.out.f16 + (size_t)f * a.total_cells + d.cell_base;
@@ -714,6 +717,14 @@ static void gg_launch3(const StepArgs& a, int grid, hipStream_t s, const hipEven
         else
             launch_sweep(gg_wave_kernel<TS, ADAPT, false, ONE>, dim3(grid), dim3(GW), s, ev, a, a.inst, a.tile_inst, a.type);
     } else {
+        // FP64 sweep: 512 threads, 2 cells each (see gg_block_kernel)
+        if constexpr (!ADAPT) {
+            if (a.ninst > 1)
+                launch_sweep(gg_block_kernel<TS, ADAPT, true, ONE, 512>, dim3(grid), dim3(512), s, ev, a, a.inst, a.tile_inst, a.type);
+            else
+                launch_sweep(gg_block_kernel<TS, ADAPT, false, ONE, 512>, dim3(grid), dim3(512), s, ev, a, a.inst, a.tile_inst, a.type);
+            return;
+        }
         if (a.ninst > 1)
             launch_sweep(gg_block_kernel<TS, ADAPT, true, ONE>, dim3(grid), dim3(WG), s, ev, a, a.inst, a.tile_inst, a.type);
         else

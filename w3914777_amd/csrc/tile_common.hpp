@@ -113,21 +113,21 @@ __device__ inline void nbfield_fill2(NbField* s0, NbField* s1, const StepArgs& a
 // Offsets (inside one instance's field array) of the box elements a thread
 // stages: element e = threadIdx.x + k*WG.  Computed once per tile and shared
 // by every field of the tile; interior tiles skip the boundary map.
-template <int TS, int H>
+template <int TS, int H, int NT = WG>
 struct BoxMap {
-    static constexpr int LW = TS + 2 * H, NE = LW * LW, N = (NE + WG - 1) / WG;
+    static constexpr int LW = TS + 2 * H, NE = LW * LW, N = (NE + NT - 1) / NT;
     int off[N];
     int reg[N];   // 3x3 region of the element (neighbour table index)
 };
 
-template <int TS, int H>
-__device__ inline void box_map(BoxMap<TS, H>& m, const InstDesc& d, int tile_x, int tile_y) {
-    constexpr int LW = TS + 2 * H, NE = LW * LW, N = BoxMap<TS, H>::N;
+template <int TS, int H, int NT = WG>
+__device__ inline void box_map(BoxMap<TS, H, NT>& m, const InstDesc& d, int tile_x, int tile_y) {
+    constexpr int LW = TS + 2 * H, NE = LW * LW, N = BoxMap<TS, H, NT>::N;
     const int ox = tile_x * TS - H, oy = tile_y * TS - H;
     const bool interior = ox >= 0 && oy >= 0 && ox + LW <= d.nx && oy + LW <= d.ny;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        const int e = threadIdx.x + k * WG;
+        const int e = threadIdx.x + k * NT;
         const int ly = e / LW, lx = e - ly * LW;
         int gy = oy + ly, gx = ox + lx;
         if (!interior) {
@@ -289,14 +289,21 @@ __device__ inline void wave_minmax(float& mn, float& mx) {
 
 // Min and max over the workgroup; red must hold 2*WG/64 floats.  Contains
 // barriers: every thread must call it.
+// NWAVES waves per workgroup; red holds 2 * NWAVES floats.
+template <int NWAVES = WG / 64>
 __device__ inline void block_minmax(float& mn, float& mx, float* red) {
     wave_minmax(mn, mx);
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     __syncthreads();
-    if (l == 0) { red[w] = mn; red[4 + w] = mx; }
+    if (l == 0) { red[w] = mn; red[NWAVES + w] = mx; }
     __syncthreads();
-    mn = fminf(fminf(red[0], red[1]), fminf(red[2], red[3]));
-    mx = fmaxf(fmaxf(red[4], red[5]), fmaxf(red[6], red[7]));
+    mn = red[0];
+    mx = red[NWAVES];
+#pragma unroll
+    for (int i = 1; i < NWAVES; ++i) {
+        mn = fminf(mn, red[i]);
+        mx = fmaxf(mx, red[NWAVES + i]);
+    }
 }
 
 // Min and max of two values over the workgroup with a single barrier; red
