@@ -681,6 +681,36 @@ __device__ inline void nb_code_store(SfLowLds<TS>& L, int slot, const NbCode& c)
     }
 }
 
+// Phase 1 of a low tile whose phi box decodes to exact zeros (the tile and
+// its 8 neighbours low, phi scale 0, every offset the tile's own): each ring
+// gradient is exactly +0, so each ring cell takes the zero-gradient values
+// (theta = 0), computed once with the operations of the loop (bitwise the
+// values it would write, signs of zero included).  Exact, not a skip.
+template <int TS, bool H2>
+__device__ inline void sf_phase1_flat(float* P, float* Q, float* E2, const InstDesc& d, float cth0, float sth0,
+                                      float epsb, float epsbd, float mend) {
+    using G = SfGeo<TS>;
+    const int n = (int)d.k[SFK_ANISO_N];
+    float eps, epsd;
+    if (H2 && n == 6) {
+        float ca, sa, ca1, sa1;
+        aniso6_h2(1.0f, 0.0f, 1.0f, 0.0f, cth0, sth0, ca, sa, ca1, sa1);
+        eps = epsb + epsbd * ca;
+        epsd = mend * sa;
+    } else {
+        sf_aniso<float>(0.0f, 0.0f, n, cth0, sth0, epsb, epsbd, mend, eps, epsd);
+    }
+    const float A = eps * epsd;
+    const float pq = A * 0.0f, e2 = eps * eps;
+    for (int e = threadIdx.x; e < G::RW * G::RW; e += WG) {
+        const int ry0 = e / G::RW;
+        const int o = e + 2 * ry0 + G::RB - 1;  // RB + ry0 PW + rx0 - 1 (PW = RW + 2)
+        P[o] = pq;
+        Q[o] = pq;
+        E2[o] = e2;
+    }
+}
+
 // One low tile.  Barriers: A (box staged, bulk-hold flags), B (phase 1
 // done; interface tiles only) and C (min/max of the new values).  The
 // neighbour table of the next tile (`nxt`) is stored into slot `slot ^ 1`
@@ -787,7 +817,18 @@ __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const 
         return;
     }
 
-    if constexpr (H2)
+    // NOSKIP: a phi box of exact zeros needs no ring gradients (see
+    // sf_phase1_flat); the codes of the 9 tiles are in LDS since barrier A
+    bool flat = false;
+    if (NOSKIP && hmask == 0u) {
+        const int l = threadIdx.x & 63;
+        const bool same = l >= 9 || (L.nbs[slot][0][l] == 0.0f && L.nbo[slot][0][l] == offp);
+        flat = __ballot(!same) == 0ull;
+    }
+    if (flat)
+        sf_phase1_flat<TS, H2>(L.P, L.Q, L.E2, d, d.kf[SFK_COS_N_THETA0], d.kf[SFK_SIN_N_THETA0], d.kf[SFK_EPSB],
+                               d.kf[SFK_EPSB_DELTA], d.kf[SFK_MEPSB_ANISO_DELTA]);
+    else if constexpr (H2)
         sf_phase1_h2<TS>(L.phi, L.P, L.Q, L.E2, d, ox, oy, d.kf[SFK_INV_2DX], d.kf[SFK_INV_2DY],
                          d.kf[SFK_COS_N_THETA0], d.kf[SFK_SIN_N_THETA0], d.kf[SFK_EPSB], d.kf[SFK_EPSB_DELTA],
                          d.kf[SFK_MEPSB_ANISO_DELTA]);
