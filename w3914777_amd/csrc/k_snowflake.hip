@@ -825,6 +825,45 @@ __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const 
         const bool same = l >= 9 || (L.nbs[slot][0][l] == 0.0f && L.nbo[slot][0][l] == offp);
         flat = __ballot(!same) == 0ull;
     }
+    // A flat box at phi exactly 0 or 1 (pa(1 - pa) = +-0 for every cell):
+    // the flux is +0 (differences of equal P, Q and a Laplacian of zeros),
+    // pa(1 - pa)(pa - 1/2 + m) and the noise term (not taken: |pp| = 0) are
+    // +-0, so br = +0 and phi' = ph + dtt br = +0; T' takes the update of
+    // the loop with phi' - ph = +0 - ph.  Same values bitwise, without the
+    // ring, the flux and the atan.
+    const float phi_off = (float)offp;
+    if (flat && (phi_off == 0.0f || phi_off == 1.0f)) {
+        float rp[R], rt[R], lt[R];
+        sf_lap<TS, float>(L.T, ly, x0, d.stencil == AMPS_STENCIL_REF, d.kf[SFK_INV_DXDY], tc, lt);
+        const float dt = d.kf[SFK_DT], kap = d.kf[SFK_KAPPA];
+        float mnt = INFINITY, mxt = -INFINITY;
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const float ph = pc[i], t = tc[i], phn = 0.0f;
+            rp[i] = phn;
+            rt[i] = t + dt * lt[i] + kap * (phn - ph);
+            mnt = fminf(mnt, rt[i]);
+            mxt = fmaxf(mxt, rt[i]);
+        }
+        float mnp = 0.0f, mxp = 0.0f;
+        block_minmax2(mnp, mxp, mnt, mxt, L.red);  // C
+        const TileCode cp = tile_code(mnp, mxp);
+        const TileCode ct = tile_code(mnt, mxt);
+        const double offpn = offp + (double)cp.mid, offTn = offT + (double)ct.mid;
+        unsigned short w[R];
+        sf_encode<R>(rp, cp, w);
+        sf_store_low<R>(dphi, gcell0, w);
+        sf_encode<R>(rt, ct, w);
+        sf_store_low<R>(dT, gcell0, w);
+        if (threadIdx.x == 0) {
+            a.out.off[gtile] = offpn;
+            a.out.scl[gtile] = cp.scl;
+            a.out.off[a.total_tiles + gtile] = offTn;
+            a.out.scl[a.total_tiles + gtile] = ct.scl;
+            a.same_flags[gtile] = 0;
+        }
+        return;
+    }
     if (flat)
         sf_phase1_flat<TS, H2>(L.P, L.Q, L.E2, d, d.kf[SFK_COS_N_THETA0], d.kf[SFK_SIN_N_THETA0], d.kf[SFK_EPSB],
                                d.kf[SFK_EPSB_DELTA], d.kf[SFK_MEPSB_ANISO_DELTA]);
