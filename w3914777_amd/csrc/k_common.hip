@@ -588,11 +588,15 @@ __global__ void clear_changed_kernel(const int* __restrict__ type_old, const int
     }
 }
 
-// Work-lists of FP64 (type 2) and low (type 1) tiles in tile order (group 0
-// types).  Workgroup j owns tiles [1024 j, 1024 j + 1024): it counts the
-// entries of all earlier tiles itself (vector loads, a block sum), so no
-// workgroup waits on another, then writes its own entries with wave ballots
-// (consecutive lanes, consecutive tiles: runs of coalesced stores).  The last
+// Work-lists of FP64 (type 2) and low (type 1) tiles (group 0 types).  The
+// FP64 entry of rank r in tile order sits at xcd_unmap(r, n), so the
+// workgroup that reads entry b (dealt to XCD b mod 8) gets rank
+// xcd_remap(b, n): each XCD a contiguous run of tiles, with no count needed
+// to find the entry.  The low list is in tile order.  Workgroup j owns
+// tiles [1024 j, 1024 j + 1024): it counts the entries of all earlier
+// tiles (and the FP64 entries of later ones) itself (vector loads, a block
+// sum), so no workgroup waits on another, then writes its own entries with
+// wave ballots.  The last
 // workgroup records the totals.  Also clears same_flags (when given) of
 // every tile whose type changed (type_old given: the tiles a monitor pass
 // converted) or of every tile (type_old null: fields or types were set from

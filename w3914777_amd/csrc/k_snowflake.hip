@@ -446,9 +446,10 @@ __device__ inline void sf_halo_of(SfHalo<TS>& hm, const InstDesc& d, int tile_x,
     }
 }
 
+// phi = false: the T elements only (hp untouched).
 template <int TS, bool ADAPT, int NT = WG>
 __device__ inline void sf_halo_load(uint2* hp, uint2* ht, const StepArgs& a, const InstDesc& d,
-                                    const SfHalo<TS, NT>& hm, unsigned hmask) {
+                                    const SfHalo<TS, NT>& hm, unsigned hmask, bool phi = true) {
     const double* fphi = a.in.f64 + d.cell_base;
     const double* fT = a.in.f64 + a.total_cells + d.cell_base;
     const unsigned short* hphi = reinterpret_cast<const unsigned short*>(a.in.f16 + d.cell_base);
@@ -458,7 +459,9 @@ __device__ inline void sf_halo_load(uint2* hp, uint2* ht, const StepArgs& a, con
         const int off = hm.off[k];
         if (off < 0) continue;
         const bool h64 = !ADAPT || ((hmask >> hm.reg[k]) & 1u);
-        hp[k] = *(h64 ? reinterpret_cast<const uint2*>(fphi + off) : reinterpret_cast<const uint2*>(hphi + (off & ~3)));
+        if (phi)
+            hp[k] = *(h64 ? reinterpret_cast<const uint2*>(fphi + off)
+                          : reinterpret_cast<const uint2*>(hphi + (off & ~3)));
         ht[k] = *(h64 ? reinterpret_cast<const uint2*>(fT + off) : reinterpret_cast<const uint2*>(hT + (off & ~3)));
     }
 }
@@ -593,10 +596,12 @@ struct LowFetch {
     float sclp, sclT;
 };
 
-// tc (may be null): launch constants of a one-instance grid.
+// tc (may be null): launch constants of a one-instance grid.  phi = false:
+// no phi words (f.op and f.hp left unset; the tile's offset and scale are
+// still loaded).
 template <int TS>
 __device__ inline void low_fetch(LowFetch<TS>& f, const StepArgs& a, const InstDesc& d, int w, unsigned hmask,
-                                 const SfTileConst<TS>* tc = nullptr) {
+                                 const SfTileConst<TS>* tc = nullptr, bool phi = true) {
     using G = SfGeo<TS>;
     constexpr int R = G::R;
     const int lt = w - d.tile_base;
@@ -607,17 +612,22 @@ __device__ inline void low_fetch(LowFetch<TS>& f, const StepArgs& a, const InstD
     const unsigned short* hphi = reinterpret_cast<const unsigned short*>(a.in.f16 + d.cell_base);
     const unsigned short* hT = reinterpret_cast<const unsigned short*>(a.in.f16 + a.total_cells + d.cell_base);
     if (R == 4) {
-        const uint2 u = *reinterpret_cast<const uint2*>(hphi + gcell0);
+        if (phi) {
+            const uint2 u = *reinterpret_cast<const uint2*>(hphi + gcell0);
+            f.op[0] = u.x & 0xFFFFu; f.op[1 % R] = u.x >> 16; f.op[2 % R] = u.y & 0xFFFFu; f.op[3 % R] = u.y >> 16;
+        }
         const uint2 v = *reinterpret_cast<const uint2*>(hT + gcell0);
-        f.op[0] = u.x & 0xFFFFu; f.op[1 % R] = u.x >> 16; f.op[2 % R] = u.y & 0xFFFFu; f.op[3 % R] = u.y >> 16;
         f.ot[0] = v.x & 0xFFFFu; f.ot[1 % R] = v.x >> 16; f.ot[2 % R] = v.y & 0xFFFFu; f.ot[3 % R] = v.y >> 16;
     } else {
 #pragma unroll
-        for (int j = 0; j < R; ++j) { f.op[j] = hphi[gcell0 + j]; f.ot[j] = hT[gcell0 + j]; }
+        for (int j = 0; j < R; ++j) {
+            if (phi) f.op[j] = hphi[gcell0 + j];
+            f.ot[j] = hT[gcell0 + j];
+        }
     }
     SfHalo<TS> hm;
     sf_halo_of<TS>(hm, d, tile_x, tile_y, tc);
-    sf_halo_load<TS, true>(f.hp, f.ht, a, d, hm, hmask);
+    sf_halo_load<TS, true>(f.hp, f.ht, a, d, hm, hmask, phi);
     f.offp = a.in.off[w];
     f.offT = a.in.off[a.total_tiles + w];
     f.sclp = a.in.scl[w];
@@ -681,6 +691,69 @@ __device__ inline void nb_code_store(SfLowLds<TS>& L, int slot, const NbCode& c)
     }
 }
 
+// How flat a low tile's neighbourhood is, from the codes of its 9 tiles
+// (lane k of every wave holds tile k, k = 4 the tile itself); uniform over
+// the workgroup.  0: none; 1: the phi box decodes to exact zeros (the 9
+// tiles low, phi scale 0, one phi offset); 2: and that offset is 0 or 1 as
+// a float; 3: and the T box decodes to exact zeros too (T scale 0, one T
+// offset).
+__device__ inline int low_flat_level(const NbCode& c, unsigned hmask) {
+    if (hmask != 0u) return 0;
+    const int l = threadIdx.x & 63;
+    const double op = __shfl(c.o0, 4), oT = __shfl(c.o1, 4);
+    const bool fp = l >= 9 || (c.s0 == 0.0f && c.o0 == op);
+    const bool fT = l >= 9 || (c.s1 == 0.0f && c.o1 == oT);
+    if (__ballot(!fp) != 0ull) return 0;
+    const float po = (float)op;
+    if (po != 0.0f && po != 1.0f) return 1;
+    return __ballot(!fT) == 0ull ? 3 : 2;
+}
+
+// A level-3 tile: the update of a level-2 tile (sf_tile_low) with the T
+// Laplacian, T and phi all signed zeros gives T' = (+-0) + kap (+0 - ph) =
+// +0 and phi' = +0 in every cell, so the new words and codes follow from
+// the tile's offsets alone (those of tile_code(+0, +0), as the min/max of
+// the new values would give): no word is read.
+// NT threads cover the cells of the WG threads of sf_tile_low (virtual
+// thread v = threadIdx.x + k NT).
+template <int TS, int NT = WG>
+__device__ inline void sf_tile_zero(const StepArgs& a, const InstDesc& d, int gtile, double offp, double offT) {
+    using G = SfGeo<TS>;
+    constexpr int R = G::R;
+    static_assert(WG % NT == 0, "whole passes");
+    const int lt = gtile - d.tile_base;
+    const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
+    const TileCode z = tile_code(0.0f, 0.0f);
+    float zr[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) zr[i] = 0.0f;
+    unsigned short w[R];
+    sf_encode<R>(zr, z, w);
+#pragma unroll
+    for (int k = 0; k < WG / NT; ++k) {
+        const int v = threadIdx.x + k * NT;
+        const int ly = v / G::TPR, x0 = (v % G::TPR) * R;
+        const int gcell0 = (tile_y * TS + ly) * d.nx + tile_x * TS + x0;
+        sf_store_low<R>(a.out.f16 + d.cell_base, gcell0, w);
+        sf_store_low<R>(a.out.f16 + a.total_cells + d.cell_base, gcell0, w);
+    }
+    if (threadIdx.x == 0) {
+        a.out.off[gtile] = offp + (double)z.mid;
+        a.out.scl[gtile] = z.scl;
+        a.out.off[a.total_tiles + gtile] = offT + (double)z.mid;
+        a.out.scl[a.total_tiles + gtile] = z.scl;
+        a.same_flags[gtile] = 0;
+    }
+}
+
+// T' of a cell of a level-2 tile (sf_tile_low, sf_flat_kernel): the update
+// of sf_phase2 with phi' = +0 (one expression for both kernels, so both
+// compile it alike).
+__device__ inline float sf_t_flat(float t, float lt, float dt, float kap, float ph) {
+    const float phn = 0.0f;
+    return t + dt * lt + kap * (phn - ph);
+}
+
 // Phase 1 of a low tile whose phi box decodes to exact zeros (the tile and
 // its 8 neighbours low, phi scale 0, every offset the tile's own): each ring
 // gradient is exactly +0, so each ring cell takes the zero-gradient values
@@ -724,9 +797,11 @@ __device__ inline void sf_phase1_flat(float* P, float* Q, float* E2, const InstD
 template <int TS, bool NOSKIP, bool H2 = false>
 __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const StepArgs& a, const InstDesc& d,
                             int gtile, const LowFetch<TS>& f, unsigned hmask, long long step,
-                            const SfTileConst<TS>* tcon = nullptr) {
+                            const SfTileConst<TS>* tcon = nullptr, int flat = 0) {
     using G = SfGeo<TS>;
     constexpr int H = G::H, LW = G::LW, R = G::R;
+    // phi words fetched (low_flat_level 2 and 3 tiles come without them)
+    const bool pw = !NOSKIP || flat < 2;
     const int lt = gtile - d.tile_base;
     const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
     const int ox = tile_x * TS, oy = tile_y * TS;
@@ -737,7 +812,7 @@ __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const 
     float pc[R], tc[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-        pc[j] = __half2float(__ushort_as_half(f.op[j])) * sclp;
+        pc[j] = pw ? __half2float(__ushort_as_half(f.op[j])) * sclp : 0.0f;
         tc[j] = __half2float(__ushort_as_half(f.ot[j])) * sclT;
     }
     SfHalo<TS> hm;
@@ -746,7 +821,7 @@ __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const 
     bool near0 = true, near1 = true;
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-        L.phi[(ly + H) * LW + x0 + H + j] = pc[j];
+        if (pw) L.phi[(ly + H) * LW + x0 + H + j] = pc[j];
         L.T[(ly + H) * LW + x0 + H + j] = tc[j];
         if (!NOSKIP) {
             near0 = near0 && fabsf(c0 + pc[j]) <= SF_BULK_EPS;
@@ -762,13 +837,16 @@ __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const 
             vp = (float)(raw_f64(f.hp[k].x, f.hp[k].y) - offp);
             L.T[hm.box[k]] = (float)(raw_f64(f.ht[k].x, f.ht[k].y) - offT);
         } else {
-            const unsigned short wp = (unsigned short)raw_word4(f.hp[k], hm.off[k]);
             const unsigned short wt = (unsigned short)raw_word4(f.ht[k], hm.off[k]);
-            vp = (float)(L.nbo[slot][0][r] - offp) + __half2float(__ushort_as_half(wp)) * L.nbs[slot][0][r];
+            vp = 0.0f;
+            if (pw) {
+                const unsigned short wp = (unsigned short)raw_word4(f.hp[k], hm.off[k]);
+                vp = (float)(L.nbo[slot][0][r] - offp) + __half2float(__ushort_as_half(wp)) * L.nbs[slot][0][r];
+            }
             L.T[hm.box[k]] =
                 (float)(L.nbo[slot][1][r] - offT) + __half2float(__ushort_as_half(wt)) * L.nbs[slot][1][r];
         }
-        L.phi[hm.box[k]] = vp;
+        if (pw) L.phi[hm.box[k]] = vp;
         if (!NOSKIP) {
             near0 = near0 && fabsf(c0 + vp) <= SF_BULK_EPS;
             near1 = near1 && fabsf(c1 - vp) <= SF_BULK_EPS;
@@ -817,31 +895,23 @@ __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const 
         return;
     }
 
-    // NOSKIP: a phi box of exact zeros needs no ring gradients (see
-    // sf_phase1_flat); the codes of the 9 tiles are in LDS since barrier A
-    bool flat = false;
-    if (NOSKIP && hmask == 0u) {
-        const int l = threadIdx.x & 63;
-        const bool same = l >= 9 || (L.nbs[slot][0][l] == 0.0f && L.nbo[slot][0][l] == offp);
-        flat = __ballot(!same) == 0ull;
-    }
-    // A flat box at phi exactly 0 or 1 (pa(1 - pa) = +-0 for every cell):
-    // the flux is +0 (differences of equal P, Q and a Laplacian of zeros),
-    // pa(1 - pa)(pa - 1/2 + m) and the noise term (not taken: |pp| = 0) are
-    // +-0, so br = +0 and phi' = ph + dtt br = +0; T' takes the update of
-    // the loop with phi' - ph = +0 - ph.  Same values bitwise, without the
-    // ring, the flux and the atan.
-    const float phi_off = (float)offp;
-    if (flat && (phi_off == 0.0f || phi_off == 1.0f)) {
+    // A flat box at phi exactly 0 or 1 (low_flat_level 2; pa(1 - pa) = +-0
+    // for every cell): the flux is +0 (differences of equal P, Q and a
+    // Laplacian of zeros), pa(1 - pa)(pa - 1/2 + m) and the noise term (not
+    // taken: |pp| = 0) are +-0, so br = +0 and phi' = ph + dtt br = +0; T'
+    // takes the update of the loop with phi' - ph = +0 - ph = +0 for either
+    // sign of ph (which is why the phi words need not be fetched: ph is
+    // taken as +0).  Same values bitwise, without the ring, the flux and
+    // the atan.
+    if (NOSKIP && flat >= 2) {
         float rp[R], rt[R], lt[R];
         sf_lap<TS, float>(L.T, ly, x0, d.stencil == AMPS_STENCIL_REF, d.kf[SFK_INV_DXDY], tc, lt);
         const float dt = d.kf[SFK_DT], kap = d.kf[SFK_KAPPA];
         float mnt = INFINITY, mxt = -INFINITY;
 #pragma unroll
         for (int i = 0; i < R; ++i) {
-            const float ph = pc[i], t = tc[i], phn = 0.0f;
-            rp[i] = phn;
-            rt[i] = t + dt * lt[i] + kap * (phn - ph);
+            rp[i] = 0.0f;
+            rt[i] = sf_t_flat(tc[i], lt[i], dt, kap, pc[i]);
             mnt = fminf(mnt, rt[i]);
             mxt = fmaxf(mxt, rt[i]);
         }
@@ -864,7 +934,8 @@ __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const 
         }
         return;
     }
-    if (flat)
+    // a phi box of exact zeros needs no ring gradients (sf_phase1_flat)
+    if (NOSKIP && flat == 1)
         sf_phase1_flat<TS, H2>(L.P, L.Q, L.E2, d, d.kf[SFK_COS_N_THETA0], d.kf[SFK_SIN_N_THETA0], d.kf[SFK_EPSB],
                                d.kf[SFK_EPSB_DELTA], d.kf[SFK_MEPSB_ANISO_DELTA]);
     else if constexpr (H2)
@@ -1262,11 +1333,12 @@ __global__ __launch_bounds__(NT, 3) void sf_high_kernel(StepArgs a, const InstDe
     // tiles updated (adaptive steps; one writer per launch)
     if (ADAPT && a.stats && b == 0 && threadIdx.x == 0) a.stats[2] += (unsigned long long)n;
     if (b >= n) return;
-    int w = wl ? wl[xcd_remap(b, n)] : xcd_remap(b, n);
+    // list entries are placed for XCD runs (worklist_kernel): read in order
+    int w = wl ? wl[b] : xcd_remap(b, n);
     int tcur = ADAPT ? nb_type_lane<TS>(a, inst[MULTI ? tinst[w] : 0], w) : 2;
     for (; b < n; b += G) {
         const bool more = b + G < n;
-        const int wn = more ? (wl ? wl[xcd_remap(b + G, n)] : xcd_remap(b + G, n)) : w;
+        const int wn = more ? (wl ? wl[b + G] : xcd_remap(b + G, n)) : w;
         const int tnext = ADAPT && more ? nb_type_lane<TS>(a, inst[MULTI ? tinst[wn] : 0], wn) : 2;
         const InstDesc& d = inst[MULTI ? tinst[w] : 0];
         const int lt = w - d.tile_base;
@@ -1312,10 +1384,11 @@ union SfMixLds {
 };
 
 //
-// NOSKIP (no hold sweep runs): the low items are the whole low work-list
-// (fseg = the list, fcnt2 = its count), taken XCD-remapped like the FP64
-// sweep's tiles, and every one runs the full low update.
-template <int TS, bool MULTI, bool NOSKIP>
+// NOSKIP: the low items are the tiles the flat sweep (sf_flat_kernel)
+// appended to this step parity's list (full_seg; counts full_cnt[0] and
+// full_cnt[8]), levels 0 and 1 of low_flat_level, each updated by the full
+// low path (H2: its half2 form).
+template <int TS, bool MULTI, bool NOSKIP, bool H2 = false>
 __global__ __launch_bounds__(256, 3) void sf_mixed_kernel(StepArgs a, const InstDesc* __restrict__ inst,
                                                       const int* __restrict__ tinst, const int* __restrict__ wl,
                                                       const int* __restrict__ wlc, const int* __restrict__ fseg,
@@ -1329,7 +1402,8 @@ __global__ __launch_bounds__(256, 3) void sf_mixed_kernel(StepArgs a, const Inst
     const int wfirst = wl[blockIdx.x < (unsigned)T ? (int)blockIdx.x : T - 1];
     int c0[8], c1[8];
     if (NOSKIP) {
-        c0[0] = c1[0] = *fcnt2;
+        c0[0] = fcnt2[0];
+        c1[0] = fcnt2[8];
     } else {
 #pragma unroll
         for (int k = 0; k < 8; ++k) { c0[k] = fcnt2[k]; c1[k] = fcnt2[8 + k]; }
@@ -1339,7 +1413,7 @@ __global__ __launch_bounds__(256, 3) void sf_mixed_kernel(StepArgs a, const Inst
     const bool odd = (step & 1) != 0;
     int cnt[8], nf = 0;
     if (NOSKIP) {
-        nf = c0[0];
+        nf = odd ? c1[0] : c0[0];
     } else {
 #pragma unroll
         for (int k = 0; k < 8; ++k) { cnt[k] = odd ? c1[k] : c0[k]; nf += cnt[k]; }
@@ -1348,7 +1422,7 @@ __global__ __launch_bounds__(256, 3) void sf_mixed_kernel(StepArgs a, const Inst
     asm volatile("" ::"s"(wfirst));  // the entry's load stays in the counts' round (not sunk into the loop)
     if (blockIdx.x == 0 && threadIdx.x == 0) {  // tile counts of the step (one writer per launch)
         a.stats[2] += (unsigned long long)nh;
-        a.stats[3] += (unsigned long long)nf;
+        if (!NOSKIP) a.stats[3] += (unsigned long long)nf;  // NOSKIP: the flat sweep counts every low tile
     }
     for (int i = blockIdx.x; i < n; i += G) {
         if (i < nh) {
@@ -1361,7 +1435,7 @@ __global__ __launch_bounds__(256, 3) void sf_mixed_kernel(StepArgs a, const Inst
         } else {
             int w;
             if (NOSKIP) {
-                w = fseg[xcd_remap(i - nh, nf)];
+                w = fseg[i - nh];
             } else {
                 // entry j of the concatenated segments
                 int j = i - nh, seg = 0;
@@ -1379,45 +1453,195 @@ __global__ __launch_bounds__(256, 3) void sf_mixed_kernel(StepArgs a, const Inst
             low_fetch<TS>(f, a, d, w, hmask);
             nb_code_store<TS>(U.l, 0, c0);
             __syncthreads();  // neighbour table visible
-            sf_tile_low<TS, NOSKIP>(U.l, 0, c0, a, d, w, f, hmask, step);
+            sf_tile_low<TS, NOSKIP, H2>(U.l, 0, c0, a, d, w, f, hmask, step, nullptr,
+                                        NOSKIP ? low_flat_level(c0, hmask) : 0);
         }
         __syncthreads();  // the LDS union is rewritten by the next item
     }
 }
 
 // Full low sweep of the NO_SKIP step: one workgroup per tile of the low
-// work-list (sf_tile_low<TS, true>).  The grid is the list's upper bound, the
-// tile count, and workgroups past the list's length exit at once.  With no
-// loop over tiles no per-tile constant stays live into the next tile (a
-// persistent loop with a software prefetch of the next tile kept them in
-// SGPRs spilled to VGPR lanes, needed 92 VGPRs against 48 here and ran 1.2x
-// slower), and the 6 resident workgroups of a CU (LDS-bound) hide each
-// other's load round.  Tile k's neighbour types are
-// loaded first (the halo addresses need them); the tile's own words, its
-// neighbours' codes and the halo then load in one round.
+// work-list (sf_tile_low<TS, true>), in tile order.  The grid is the list's
+// upper bound, the tile count, and workgroups past the list's length exit
+// at once.  Workgroup b takes entry b: the workgroups, which the hardware
+// deals round-robin over the 8 XCDs, spread the costly tiles near the
+// crystal evenly over them (contiguous runs per XCD left the XCDs holding
+// the crystal's rows working 1.5x longer than those of the far field).
+// With no loop over tiles no per-tile constant stays live into the next
+// tile (a persistent loop with a software prefetch of the next tile kept
+// them in SGPRs spilled to VGPR lanes, needed 92 VGPRs against 48 here and
+// ran 1.2x slower), and the 6 resident workgroups of a CU (LDS-bound) hide
+// each other's load rounds.  The neighbour types and codes load first
+// (L2-resident arrays; the codes say how flat the neighbourhood is,
+// low_flat_level); a level-3 tile (sf_tile_zero) then loads nothing more,
+// a level-2 tile only its T words, other tiles phi and T.  (Loading the T
+// words with the codes made every level-3 workgroup wait for HBM loads it
+// does not use.)
 template <int TS, bool MULTI, bool H2>
 __global__ __launch_bounds__(256) void sf_low_tile_kernel(StepArgs a, const InstDesc* __restrict__ inst,
                                                           const int* __restrict__ tinst, const int* __restrict__ wl,
                                                           const int* __restrict__ wlc) {
     __shared__ __align__(16) SfLowLds<TS> L;
-    const int n = *wlc, b = blockIdx.x;
+    const int b = blockIdx.x;
+    // entry b loads beside the count: the grid is the list's capacity, so the
+    // load stays in bounds past the count
+    const int w = wl[b];
+    const int n = *wlc;
     if (b == 0 && threadIdx.x == 0) a.stats[3] += (unsigned long long)n;  // one writer
     if (b >= n) return;
     const long long step = *a.step_ctr + a.step_add;
-    const int w = wl[xcd_remap(b, n)];
     SfTileConst<TS> tconst;
     if (!MULTI) sf_tile_const<TS>(tconst, inst[0]);
     const SfTileConst<TS>* tc = MULTI ? nullptr : &tconst;
     const InstDesc& d = inst[MULTI ? tinst[w] : 0];
-    const unsigned hm = hmask_of(nb_type_lane<TS>(a, d, w, tc));
+    const int tt = nb_type_lane<TS>(a, d, w, tc);
     NbCode c;
     nb_code_fetch<TS>(c, a, d, w, tc);
+    const unsigned hm = hmask_of(tt);
+    const int flat = low_flat_level(c, hm);
+    if (flat == 3) {
+        sf_tile_zero<TS>(a, d, w, __shfl(c.o0, 4), __shfl(c.o1, 4));
+        return;
+    }
     LowFetch<TS> f;
-    low_fetch<TS>(f, a, d, w, hm, tc);
+    low_fetch<TS>(f, a, d, w, hm, tc, flat < 2);
     nb_code_store<TS>(L, 0, c);
     __syncthreads();  // neighbour table visible
     // (the tile stores `c` again into slot 1, which nothing reads)
-    sf_tile_low<TS, true, H2>(L, 0, c, a, d, w, f, hm, step, tc);
+    sf_tile_low<TS, true, H2>(L, 0, c, a, d, w, f, hm, step, tc, flat);
+}
+
+// ------------------------------------------------------------- flat sweep
+// First sweep of the NO_SKIP adaptive step: one wave per tile of the low
+// work-list (workgroup b takes entry b, tile order, as sf_low_tile_kernel
+// does).  Tiles of low_flat_level 3 and 2 are finished here, with the
+// values of sf_tile_zero and of sf_tile_low's level-2 path bitwise; a tile
+// of level 0 or 1 is appended to this step parity's list (full_seg, count
+// full_cnt[8 parity]) for the mixed sweep, which updates it beside the
+// FP64 tiles.  A level-2 tile needs only its T box in LDS (5 KB at tile
+// 32): about 30 of these workgroups fit a CU against 6 of the full low
+// tile (25 KB), so that many more tiles' load rounds overlap.  The wave
+// covers a tile as the WG threads of sf_tile_low would, in WG/64 passes
+// (virtual thread v = lane + 64 k); neighbour codes come from lanes 0..8 by
+// cross-lane reads, and the tile's min/max is a wave reduction.
+constexpr int FLAT_NT = 64;
+
+template <int TS, bool MULTI>
+__global__ __launch_bounds__(FLAT_NT) void sf_flat_kernel(StepArgs a, const InstDesc* __restrict__ inst,
+                                                          const int* __restrict__ tinst, const int* __restrict__ wl,
+                                                          const int* __restrict__ wlc) {
+    using G = SfGeo<TS>;
+    using GF = SfGeo<TS, FLAT_NT>;
+    constexpr int H = G::H, LW = G::LW, R = G::R, NP = WG / FLAT_NT;
+    __shared__ __align__(16) float Tb[LW * LW];
+    const int b = blockIdx.x;
+    const int w = wl[b];
+    const int n = *wlc;
+    const long long step = *a.step_ctr + a.step_add;
+    if (b == 0 && threadIdx.x == 0) {
+        a.stats[3] += (unsigned long long)n;  // low tiles of the step (one writer)
+        // the next step's list: its last reader, the previous step's mixed sweep, has finished
+        a.full_cnt[(int)((step + 1) & 1) * 8] = 0;
+    }
+    if (b >= n) return;
+    const InstDesc& d = inst[MULTI ? tinst[w] : 0];
+    const int tt = nb_type_lane<TS>(a, d, w);
+    NbCode c;
+    nb_code_fetch<TS>(c, a, d, w);
+    const unsigned hm = hmask_of(tt);
+    const int flat = low_flat_level(c, hm);
+    const double offp = __shfl(c.o0, 4), offT = __shfl(c.o1, 4);  // lane 4: the tile itself
+    if (flat == 3) {
+        sf_tile_zero<TS, FLAT_NT>(a, d, w, offp, offT);
+        return;
+    }
+    if (flat < 2) {
+        if (threadIdx.x == 0) a.full_seg[atomicAdd(a.full_cnt + (int)(step & 1) * 8, 1)] = w;
+        return;
+    }
+    // level 2: the T words of the tile and of its halo (no FP64 neighbour:
+    // every element from the fp16 storage)
+    const int lt = w - d.tile_base;
+    const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
+    const int ox = tile_x * TS, oy = tile_y * TS;
+    const unsigned short* hT = reinterpret_cast<const unsigned short*>(a.in.f16 + a.total_cells + d.cell_base);
+    unsigned short ot[NP][R];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const int v = threadIdx.x + k * FLAT_NT;
+        const int gcell0 = (oy + v / G::TPR) * d.nx + ox + (v % G::TPR) * R;
+        if (R == 4) {
+            const uint2 u = *reinterpret_cast<const uint2*>(hT + gcell0);
+            ot[k][0] = u.x & 0xFFFFu; ot[k][1 % R] = u.x >> 16; ot[k][2 % R] = u.y & 0xFFFFu; ot[k][3 % R] = u.y >> 16;
+        } else {
+#pragma unroll
+            for (int j = 0; j < R; ++j) ot[k][j] = hT[gcell0 + j];
+        }
+    }
+    SfHalo<TS, FLAT_NT> hmap;
+    sf_halo_map<TS, FLAT_NT>(hmap, d, ox, oy);
+    uint2 hp[GF::NHT], ht[GF::NHT];
+    sf_halo_load<TS, true, FLAT_NT>(hp, ht, a, d, hmap, 0u, false);
+    const float sclT = __shfl(c.s1, 4);
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const int v = threadIdx.x + k * FLAT_NT;
+        const int ly = v / G::TPR, x0 = (v % G::TPR) * R;
+#pragma unroll
+        for (int j = 0; j < R; ++j)
+            Tb[(ly + H) * LW + x0 + H + j] = __half2float(__ushort_as_half(ot[k][j])) * sclT;
+    }
+#pragma unroll
+    for (int k = 0; k < GF::NHT; ++k) {
+        const int r = hmap.reg[k];
+        const double o1 = __shfl(c.o1, r);  // every lane takes part in the cross-lane reads
+        const float s1 = __shfl(c.s1, r);
+        if (hmap.off[k] < 0) continue;
+        const unsigned short wt = (unsigned short)raw_word4(ht[k], hmap.off[k]);
+        Tb[hmap.box[k]] = (float)(o1 - offT) + __half2float(__ushort_as_half(wt)) * s1;
+    }
+    __syncthreads();  // the T box is in LDS
+    const float dt = d.kf[SFK_DT], kap = d.kf[SFK_KAPPA];
+    float rt[NP][R];
+    float mnt = INFINITY, mxt = -INFINITY;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const int v = threadIdx.x + k * FLAT_NT;
+        const int ly = v / G::TPR, x0 = (v % G::TPR) * R;
+        float tc[R], lp[R];
+        sf_lap<TS, float>(Tb, ly, x0, d.stencil == AMPS_STENCIL_REF, d.kf[SFK_INV_DXDY], tc, lp);
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            rt[k][i] = sf_t_flat(tc[i], lp[i], dt, kap, 0.0f);
+            mnt = fminf(mnt, rt[k][i]);
+            mxt = fmaxf(mxt, rt[k][i]);
+        }
+    }
+    wave_minmax(mnt, mxt);
+    const TileCode cp = tile_code(0.0f, 0.0f), ct = tile_code(mnt, mxt);
+    float zr[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) zr[i] = 0.0f;
+    unsigned short wz[R];
+    sf_encode<R>(zr, cp, wz);
+    __half* dphi = a.out.f16 + d.cell_base;
+    __half* dT = a.out.f16 + a.total_cells + d.cell_base;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const int v = threadIdx.x + k * FLAT_NT;
+        const int gcell0 = (oy + v / G::TPR) * d.nx + ox + (v % G::TPR) * R;
+        unsigned short wt[R];
+        sf_encode<R>(rt[k], ct, wt);
+        sf_store_low<R>(dphi, gcell0, wz);
+        sf_store_low<R>(dT, gcell0, wt);
+    }
+    if (threadIdx.x == 0) {
+        a.out.off[w] = offp + (double)cp.mid;
+        a.out.scl[w] = cp.scl;
+        a.out.off[a.total_tiles + w] = offT + (double)ct.mid;
+        a.out.scl[a.total_tiles + w] = ct.scl;
+        a.same_flags[w] = 0;
+    }
 }
 
 // ------------------------------------------------------------ host launch
@@ -1444,23 +1668,25 @@ template <int TS, bool MULTI>
 static void sf_launch(const StepArgs& a, bool adaptive, hipStream_t s, const hipEvent_t* ev) {
     dim3 blk(WG);
     if (adaptive && (a.flags & AMPS_FLAG_NO_SKIP)) {
-        // the FP64 tiles (persistent: the one-tile form measured no faster
-        // on them), then every low tile, one workgroup each (no hold sweep)
-        // 32^2 tiles: 512 threads (2 cells each), as in the PURE_FP64 sweep
-        constexpr int NTH = TS == 32 ? 512 : WG;
-        constexpr auto kh = sf_high_kernel<TS, true, MULTI, NTH>;
-        launch_sweep(kh, dim3(resident_grid<kh, NTH>(a.total_tiles, a.grid_cap)), dim3(NTH), s, ev, a, a.inst,
-                     a.tile_inst, a.worklist, a.wl_count);
-        // HALF2 is validated to tile 32 (R = 4 cells per thread, in pairs)
+        // flat sweep over the low work-list (one wave per tile; it lists the
+        // tiles of low_flat_level 0 and 1), then the FP64 tiles and those
+        // low tiles in one launch; HALF2 is validated to tile 32 (R = 4
+        // cells per thread, in pairs)
+        constexpr auto kf = sf_flat_kernel<TS, MULTI>;
+        launch_sweep(kf, dim3(a.total_tiles), dim3(FLAT_NT), s, ev, a, a.inst, a.tile_inst, a.wl_low,
+                     a.wl_low_count);
         if constexpr (TS == 32) {
             if (a.flags & AMPS_FLAG_HALF2) {
-                launch_sweep(sf_low_tile_kernel<TS, MULTI, true>, dim3(a.total_tiles), blk, s, ev ? ev + 2 : nullptr,
-                             a, a.inst, a.tile_inst, a.wl_low, a.wl_low_count);
+                constexpr auto km = sf_mixed_kernel<TS, MULTI, true, true>;
+                launch_sweep(km, dim3(resident_grid<km>(a.total_tiles, a.grid_cap)), blk, s, ev ? ev + 2 : nullptr,
+                             a, a.inst, a.tile_inst, a.worklist, a.wl_count, (const int*)a.full_seg,
+                             (const int*)a.full_cnt);
                 return;
             }
         }
-        launch_sweep(sf_low_tile_kernel<TS, MULTI, false>, dim3(a.total_tiles), blk, s, ev ? ev + 2 : nullptr, a,
-                         a.inst, a.tile_inst, a.wl_low, a.wl_low_count);
+        constexpr auto km = sf_mixed_kernel<TS, MULTI, true, false>;
+        launch_sweep(km, dim3(resident_grid<km>(a.total_tiles, a.grid_cap)), blk, s, ev ? ev + 2 : nullptr, a,
+                     a.inst, a.tile_inst, a.worklist, a.wl_count, (const int*)a.full_seg, (const int*)a.full_cnt);
     } else if (adaptive) {
         // bulk-hold sweep first (it lists the low tiles it does not hold),
         // then FP64 tiles and those low tiles in one launch
