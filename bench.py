@@ -100,6 +100,7 @@ def timed_run(amps, cfg, horizon, K, R, W, fields=True, kernel_pass=False):
             i1 = ctx.info()
             res["k_fp64_tiles"] = i1.fp64_tile_updates - i0.fp64_tile_updates
             res["k_low_tiles"] = i1.low_full_tile_updates - i0.low_full_tile_updates
+            res["k_listed_tiles"] = i1.low_listed_tile_updates - i0.low_listed_tile_updates
     return res
 
 
@@ -507,20 +508,25 @@ def main():
                    "adaptive_window_event_ms": amp["event_ms"],
                    "note": "value = median of the windows; wall clock around amps_step + amps_sync"}
 
-    # roofline: the sweeps of the adaptive step, timed with HIP events bound
-    # to each dispatch over the same windows; tiles updated from the device's
-    # counters.  Algorithmic bytes: 32 B per FP64 cell, 8 B per fp16 cell.
+    # roofline: the two sweeps of the adaptive step, timed with HIP events
+    # bound to each dispatch over the same windows; tiles updated from the
+    # device's counters.  The flat sweep finishes the low tiles whose
+    # neighbourhood decodes to exact zeros at phi 0 or 1 and lists the rest;
+    # the mixed sweep updates the FP64 tiles and the listed low tiles.
+    # Algorithmic bytes: 32 B per FP64 cell, 8 B per fp16 cell.
     nl = max(amp["k_launches"], 1)
-    fp64_s, low_s = amp["k_fp64_ms"] / nl / 1e3, amp["k_low_ms"] / nl / 1e3
+    mix_s, flat_s = amp["k_fp64_ms"] / nl / 1e3, amp["k_low_ms"] / nl / 1e3
     fp64_tiles, low_tiles = amp["k_fp64_tiles"] / nl, amp["k_low_tiles"] / nl
+    listed = amp["k_listed_tiles"] / nl
     sweeps = {
-        "sf_high_kernel<32,true,false,512>": (fp64_s, fp64_tiles * tile_cells * B_FP64, fp64_tiles),
-        "sf_low_tile_kernel<32,false,false>": (low_s, low_tiles * tile_cells * B_FP16, low_tiles),
+        "sf_flat_kernel<32,false>": (flat_s, (low_tiles - listed) * tile_cells * B_FP16, low_tiles - listed),
+        "sf_mixed_kernel<32,false,true,false>": (
+            mix_s, (fp64_tiles * B_FP64 + listed * B_FP16) * tile_cells, fp64_tiles + listed),
     }
     dom = max(sweeps, key=lambda k: sweeps[k][0])
     t_dom, b_dom, n_dom = sweeps[dom]
     achieved = b_dom / t_dom / 1e9
-    tr = traffic_from_profiles("sf_low_tile_kernel" if "low" in dom else "sf_high_kernel_adaptive", workload_key,
+    tr = traffic_from_profiles("sf_flat_kernel" if "flat" in dom else "sf_mixed_kernel_nsk", workload_key,
                                amp["start_step"], amp["end_step"])
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": achieved / HBM_PEAK_GBS,
@@ -531,6 +537,9 @@ def main():
                        "traffic_source": tr["source"] if tr else None}
     out["adaptive_step_kernels_us"] = {k: v[0] * 1e6 for k, v in sweeps.items()}
     out["adaptive_step_kernels_us"]["tiles_per_step"] = {k: v[2] for k, v in sweeps.items()}
+    out["adaptive_step_kernels_us"]["fp64_tiles_per_step"] = fp64_tiles
+    out["adaptive_step_kernels_us"]["low_tiles_per_step"] = low_tiles
+    out["adaptive_step_kernels_us"]["listed_low_tiles_per_step"] = listed
     kf_s = f64["k_fp64_ms"] / max(f64["k_launches"], 1) / 1e3
     fa = B_FP64 * cells / kf_s / 1e9
     out["fp64"]["roofline"] = {"kernel": "sf_f64_tile_kernel<32,false,512>", "avg_launch_us": kf_s * 1e6,
@@ -579,8 +588,8 @@ def main():
                "phi_l2_rms": e["phi"]["l2_rms"], "T_l2_rms": e["T"]["l2_rms"], "phi_linf": e["phi"]["linf"]}
         if d == args.dilate:
             nl = max(r["k_launches"], 1)
-            row["sf_low_tile_kernel_h2_us"] = r["k_low_ms"] / nl * 1e3
-            row["sf_high_kernel_us"] = r["k_fp64_ms"] / nl * 1e3
+            row["sf_flat_kernel_us"] = r["k_low_ms"] / nl * 1e3
+            row["sf_mixed_kernel_h2_us"] = r["k_fp64_ms"] / nl * 1e3
         h2_rows.append(row)
         log(f"[bench] half2 d={d}: {v / f64_value:.2f}x, phi rms {row['phi_l2_rms']:.3g}")
     out["half2_variant"] = {

@@ -163,6 +163,8 @@ typedef struct {
                                       tiles updated, summed over steps                */
     int64_t low_full_tile_updates; /* ... and low tiles given the full update (without
                                       NO_SKIP: those outside the bulk-hold state)     */
+    int64_t low_listed_tile_updates; /* NO_SKIP: low tiles the flat sweep left to the
+                                        mixed sweep (their neighbourhood not flat)    */
 } amps_info;
 
 /* ---- configuration ---- */
@@ -277,12 +279,13 @@ int amps_set_kernel_timing(amps_ctx* ctx, int enable);
 int amps_get_kernel_time(amps_ctx* ctx, int64_t* launches, double* total_ms);
 
 /* The same split by kernel.  The snowflake's low-precision step runs two:
- * first the bulk-hold sweep over the low tiles (AMPS_SWEEP_LOW), then one
- * kernel that updates the FP64 tiles together with the low tiles outside the
- * bulk-hold state (AMPS_SWEEP_FP64); with AMPS_FLAG_NO_SKIP, first the sweep
- * of the FP64 tiles (AMPS_SWEEP_FP64), then the full sweep of every low tile
- * (AMPS_SWEEP_LOW).  For every other step AMPS_SWEEP_FP64 is the whole sweep
- * and AMPS_SWEEP_LOW is zero.  Replaces the per-kernel
+ * first a sweep over the low tiles (AMPS_SWEEP_LOW), then one kernel that
+ * updates the FP64 tiles together with the low tiles that sweep listed
+ * (AMPS_SWEEP_FP64).  The first sweep is the bulk-hold sweep (it lists the
+ * tiles outside the bulk-hold state) or, with AMPS_FLAG_NO_SKIP, the flat
+ * sweep (it finishes the tiles whose neighbourhood decodes to exact zeros
+ * at phi 0 or 1 and lists the others).  For every other step
+ * AMPS_SWEEP_FP64 is the whole sweep and AMPS_SWEEP_LOW is zero.  Replaces the per-kernel
  * cudaEvent pairs around kernel1_mix/mix_kernel2
  * (snowflake/monitor_independent_withoutmonitor.cu:85-127). */
 enum { AMPS_SWEEP_ALL = 0, AMPS_SWEEP_FP64 = 1, AMPS_SWEEP_LOW = 2 };

@@ -97,7 +97,8 @@ class Info(ctypes.Structure):
     _fields_ = [("steps_done", ctypes.c_int64), ("monitor_passes", ctypes.c_int64), ("ntiles", ctypes.c_int32),
                 ("type_groups", ctypes.c_int32), ("nfields", ctypes.c_int32), ("ninstances", ctypes.c_int32),
                 ("hi_frac_last", ctypes.c_double), ("hi_frac_mean", ctypes.c_double), ("cells", ctypes.c_double),
-                ("fp64_tile_updates", ctypes.c_int64), ("low_full_tile_updates", ctypes.c_int64)]
+                ("fp64_tile_updates", ctypes.c_int64), ("low_full_tile_updates", ctypes.c_int64),
+                ("low_listed_tile_updates", ctypes.c_int64)]
 
 
 _lib = None

@@ -118,7 +118,8 @@ struct StepArgs {
     int* full_seg;              // snowflake: [8][full_cap] low tiles outside the bulk-hold state this step
     int* full_cnt;              // [2][8] entries per segment, by step parity
     int full_cap;
-    unsigned long long* stats;  // [2] FP64 tiles, [3] low tiles updated by the mixed sweep (summed)
+    unsigned long long* stats;  // [2] FP64 tiles, [3] low tiles given the full update, [4] NO_SKIP: low
+                                // tiles the flat sweep listed (summed over steps)
     const int* type;            // [group][total_tiles] (nullptr: all FP64)
     long long total_cells;
     int total_tiles;
@@ -146,6 +147,27 @@ __device__ inline int xcd_remap(int b, int n) {
     int per = n >> 3;
     if (b < (per << 3)) return (b & 7) * per + (b >> 3);
     return b;
+}
+
+// Chunked, balanced variant for items of uneven cost: chunks of XCD_CHUNK
+// consecutive items, chunk j to XCD j mod 8, so that neighbouring items
+// share an XCD's L2 while every XCD samples the whole range.  The
+// workgroup b (dealt to XCD b mod 8, its (b / 8)-th there) gets item
+// xcd_chunk_remap(b, n); bijective on [0, n).
+constexpr int XCD_CHUNK = 12;
+__device__ inline int xcd_chunk_remap(int b, int n) {
+    const int full = n / (8 * XCD_CHUNK) * (8 * XCD_CHUNK);
+    if (b >= full) return b;
+    const int x = b & 7, q = b >> 3;
+    return ((q / XCD_CHUNK) * 8 + x) * XCD_CHUNK + q % XCD_CHUNK;
+}
+
+// Inverse of xcd_chunk_remap: the b with xcd_chunk_remap(b, n) == r.
+__device__ inline int xcd_chunk_unmap(int r, int n) {
+    const int full = n / (8 * XCD_CHUNK) * (8 * XCD_CHUNK);
+    if (r >= full) return r;
+    const int j = r / XCD_CHUNK, o = r - j * XCD_CHUNK;
+    return ((j >> 3) * XCD_CHUNK + o) * 8 + (j & 7);
 }
 
 // Inverse of xcd_remap: the b with xcd_remap(b, n) == r.

@@ -80,7 +80,7 @@ struct amps_ctx {
     double* d_md = nullptr;
     double* d_last = nullptr;
     unsigned long long* d_stats = nullptr;  // [0] sum of FP64 tiles over passes, [1] last pass,
-                                            // [2], [3] tiles of the snowflake mixed sweep (summed)
+                                            // [2], [3], [4] snowflake sweep tile counts (summed)
     long long* d_step = nullptr;
     double* d_aux[2] = {nullptr, nullptr};
     double* d_scratch = nullptr;           // max instance cells (get/set)
@@ -260,7 +260,7 @@ bool has_low(const amps_ctx* c) { return c->mode != AMPS_PURE_FP64; }
 // tiles it did not hold); with NO_SKIP the FP64 sweep, then the full low
 // sweep.
 bool two_sweeps(const amps_ctx* c) { return c->model == AMPS_SNOWFLAKE && has_low(c); }
-bool low_sweep_first(const amps_ctx* c) { return two_sweeps(c) && !(c->flags & AMPS_FLAG_NO_SKIP); }
+bool low_sweep_first(const amps_ctx* c) { return two_sweeps(c); }
 
 StepArgs step_args(amps_ctx* c, int from) {
     StepArgs a{};
@@ -870,6 +870,7 @@ int amps_get_info(amps_ctx* c, amps_info* info) {
     info->steps_done = c->steps_done;
     info->fp64_tile_updates = (int64_t)st[2];
     info->low_full_tile_updates = (int64_t)st[3];
+    info->low_listed_tile_updates = (int64_t)st[4];
     info->monitor_passes = c->monitor_passes;
     int maxt = 0;
     for (auto& d : c->desc) maxt = std::max(maxt, d.tx * d.ty);

@@ -592,11 +592,12 @@ __global__ void clear_changed_kernel(const int* __restrict__ type_old, const int
 // FP64 entry of rank r in tile order sits at xcd_unmap(r, n), so the
 // workgroup that reads entry b (dealt to XCD b mod 8) gets rank
 // xcd_remap(b, n): each XCD a contiguous run of tiles, with no count needed
-// to find the entry.  The low list is in tile order.  Workgroup j owns
-// tiles [1024 j, 1024 j + 1024): it counts the entries of all earlier
-// tiles (and the FP64 entries of later ones) itself (vector loads, a block
-// sum), so no workgroup waits on another, then writes its own entries with
-// wave ballots.  The last
+// to find the entry.  The low entry of rank r sits at xcd_chunk_unmap(r,
+// n): runs of XCD_CHUNK tiles per XCD, the runs dealt round-robin (the low
+// tiles' costs vary across the grid).  Workgroup j owns tiles
+// [1024 j, 1024 j + 1024): it counts the entries of all earlier and later
+// tiles itself (vector loads, a block sum), so no workgroup waits on
+// another, then writes its own entries with wave ballots.  The last
 // workgroup records the totals.  Also clears same_flags (when given) of
 // every tile whose type changed (type_old given: the tiles a monitor pass
 // converted) or of every tile (type_old null: fields or types were set from
@@ -607,7 +608,7 @@ __global__ __launch_bounds__(1024) void worklist_kernel(const int* __restrict__ 
                                                         int total_tiles, int* __restrict__ wl_hi,
                                                         int* __restrict__ wl_lo, int* __restrict__ counts,
                                                         int* __restrict__ same_flags) {
-    __shared__ int ws[3][16];
+    __shared__ int ws[4][16];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int t0 = blockIdx.x * WL_TILES;
     int bh = 0, bl = 0;
@@ -616,9 +617,13 @@ __global__ __launch_bounds__(1024) void worklist_kernel(const int* __restrict__ 
         bh += (v.x == 2) + (v.y == 2) + (v.z == 2) + (v.w == 2);
         bl += (v.x == 1) + (v.y == 1) + (v.z == 1) + (v.w == 1);
     }
-    // FP64 tiles after this block's chunk: the total places the entries
-    int ah = 0;
-    for (int u = t0 + WL_TILES + tid; u < total_tiles; u += WL_TILES) ah += type[u] == 2;
+    // FP64 and low tiles after this block's chunk: the totals place the entries
+    int ah = 0, al = 0;
+    for (int u = t0 + WL_TILES + tid; u < total_tiles; u += WL_TILES) {
+        const int tu = type[u];
+        ah += tu == 2;
+        al += tu == 1;
+    }
     const int t = t0 + tid;
     const int ty = t < total_tiles ? type[t] : 0;
     const bool hi = ty == 2, lo = ty == 1;
@@ -626,15 +631,17 @@ __global__ __launch_bounds__(1024) void worklist_kernel(const int* __restrict__ 
     bh = __ockl_wfred_add_i32(bh);
     bl = __ockl_wfred_add_i32(bl);
     ah = __ockl_wfred_add_i32(ah);
+    al = __ockl_wfred_add_i32(al);
     if (lane == 0) {
         ws[0][wave] = bh;
         ws[1][wave] = bl;
         ws[2][wave] = ah;
+        ws[3][wave] = al;
     }
     __syncthreads();
-    int oh = 0, ol = 0, nafter = 0;
+    int oh = 0, ol = 0, nafter = 0, lafter = 0;
 #pragma unroll
-    for (int w = 0; w < 16; ++w) { oh += ws[0][w]; ol += ws[1][w]; nafter += ws[2][w]; }
+    for (int w = 0; w < 16; ++w) { oh += ws[0][w]; ol += ws[1][w]; nafter += ws[2][w]; lafter += ws[3][w]; }
     __syncthreads();
     if (lane == 0) {
         ws[0][wave] = __popcll(mh);
@@ -650,11 +657,10 @@ __global__ __launch_bounds__(1024) void worklist_kernel(const int* __restrict__ 
         tl += y;
     }
     const unsigned long long below = (1ull << lane) - 1ull;
-    // FP64 entry of rank r (tile order) at the position q with xcd_remap(q, n) == r:
-    // consumers read wl_hi[i] for item i and still give each XCD a contiguous run
-    const int nhi = th + nafter;
+    // entry of rank r (tile order) at the position the consumers' item r maps to
+    const int nhi = th + nafter, nlo = tl + lafter;
     if (hi) wl_hi[xcd_unmap(oh + __popcll(mh & below), nhi)] = t;
-    if (lo) wl_lo[ol + __popcll(ml & below)] = t;
+    if (lo) wl_lo[xcd_chunk_unmap(ol + __popcll(ml & below), nlo)] = t;
     if (same_flags && t < total_tiles && (!type_old || type_old[t] != ty)) same_flags[t] = 0;
     if (tid == 0 && t0 + WL_TILES >= total_tiles) { counts[0] = th; counts[1] = tl; }
 }

@@ -32,6 +32,8 @@
 #include "tile_common.hpp"
 #include "sf_math.hpp"
 
+#include <type_traits>
+
 namespace amps {
 
 // Geometry of a snowflake tile of TS x TS cells updated by NT threads.
@@ -140,6 +142,23 @@ __device__ inline void sf_phase1(const Rt* phi, Rt* P, Rt* Q, Rt* E2, const Inst
 
 // Laplacian of the R cells (row ly, columns x0..x0+R-1) of a box field;
 // also returns the cells' own values.  Reads are 16-byte aligned rows.
+// Multiply-adds of the FP32 low path are written as fmaf: one rounding at
+// every site, whatever the compiler would fuse, so that the paths that
+// update the same tile (the flat sweep's and the full low update) give the
+// same bits.
+// T' = t + dt lap T + kappa dphi (FP32: fmaf(kappa, dphi, fmaf(dt, lap, t))).
+template <typename Rt>
+__device__ inline Rt sf_t_update(Rt t, Rt lt, Rt dt, Rt kap, Rt dphi) {
+    if constexpr (std::is_same<Rt, float>::value) return fmaf(kap, dphi, fmaf(dt, lt, t));
+    else return t + dt * lt + kap * dphi;
+}
+
+// A neighbour's fp16 word as a deviation from this tile's offset:
+// (float)(its offset - ours) + word * its scale.
+__device__ inline float sf_decode_rel(double doff, unsigned short w, float scl) {
+    return fmaf(__half2float(__ushort_as_half(w)), scl, (float)doff);
+}
+
 template <int TS, typename Rt, int NT = WG>
 __device__ inline void sf_lap(const Rt* F, int ly, int x0, bool ref, Rt inv_dxdy, Rt* own, Rt* lap) {
     using Gm = SfGeo<TS, NT>;
@@ -159,16 +178,21 @@ __device__ inline void sf_lap(const Rt* F, int ly, int x0, bool ref, Rt inv_dxdy
         }
         // order of snowflake/function.h:27-28: xs1, xs2, xa1, xa2, ys1, ys2, ya1, ya2
 #pragma unroll
-        for (int i = 0; i < R; ++i)
-            lap[i] = (row[i + 1] + row[i] + row[i + 3] + row[i + 4] + u1[i] + u2[i] + d1[i] + d2[i] - Rt(8) * row[i + 2]) *
-                     inv_dxdy;
+        for (int i = 0; i < R; ++i) {
+            const Rt sum = row[i + 1] + row[i] + row[i + 3] + row[i + 4] + u1[i] + u2[i] + d1[i] + d2[i];
+            if constexpr (std::is_same<Rt, float>::value) lap[i] = fmaf(-8.0f, row[i + 2], sum) * inv_dxdy;
+            else lap[i] = (sum - Rt(8) * row[i + 2]) * inv_dxdy;
+        }
     } else {
         Rt u1[R], d1[R];
 #pragma unroll
         for (int j = 0; j < R; ++j) { u1[j] = F[b - LW + H + j]; d1[j] = F[b + LW + H + j]; }
 #pragma unroll
-        for (int i = 0; i < R; ++i)
-            lap[i] = (row[i + 1] + row[i + 3] + u1[i] + d1[i] - Rt(4) * row[i + 2]) * inv_dxdy;
+        for (int i = 0; i < R; ++i) {
+            const Rt sum = row[i + 1] + row[i + 3] + u1[i] + d1[i];
+            if constexpr (std::is_same<Rt, float>::value) lap[i] = fmaf(-4.0f, row[i + 2], sum) * inv_dxdy;
+            else lap[i] = (sum - Rt(4) * row[i + 2]) * inv_dxdy;
+        }
     }
 }
 
@@ -353,7 +377,7 @@ __device__ inline void sf_phase2(const Rt* F, const Rt* G, const Rt* P, const Rt
         }
         const Rt phn = ph + dtt * br;
         np[i] = phn;
-        nt[i] = t + dt * lt[i] + kap * (phn - ph);
+        nt[i] = sf_t_update<Rt>(t, lt[i], dt, kap, phn - ph);
     }
 }
 
@@ -596,12 +620,10 @@ struct LowFetch {
     float sclp, sclT;
 };
 
-// tc (may be null): launch constants of a one-instance grid.  phi = false:
-// no phi words (f.op and f.hp left unset; the tile's offset and scale are
-// still loaded).
+// tc (may be null): launch constants of a one-instance grid.
 template <int TS>
 __device__ inline void low_fetch(LowFetch<TS>& f, const StepArgs& a, const InstDesc& d, int w, unsigned hmask,
-                                 const SfTileConst<TS>* tc = nullptr, bool phi = true) {
+                                 const SfTileConst<TS>* tc = nullptr) {
     using G = SfGeo<TS>;
     constexpr int R = G::R;
     const int lt = w - d.tile_base;
@@ -612,22 +634,17 @@ __device__ inline void low_fetch(LowFetch<TS>& f, const StepArgs& a, const InstD
     const unsigned short* hphi = reinterpret_cast<const unsigned short*>(a.in.f16 + d.cell_base);
     const unsigned short* hT = reinterpret_cast<const unsigned short*>(a.in.f16 + a.total_cells + d.cell_base);
     if (R == 4) {
-        if (phi) {
-            const uint2 u = *reinterpret_cast<const uint2*>(hphi + gcell0);
-            f.op[0] = u.x & 0xFFFFu; f.op[1 % R] = u.x >> 16; f.op[2 % R] = u.y & 0xFFFFu; f.op[3 % R] = u.y >> 16;
-        }
+        const uint2 u = *reinterpret_cast<const uint2*>(hphi + gcell0);
         const uint2 v = *reinterpret_cast<const uint2*>(hT + gcell0);
+        f.op[0] = u.x & 0xFFFFu; f.op[1 % R] = u.x >> 16; f.op[2 % R] = u.y & 0xFFFFu; f.op[3 % R] = u.y >> 16;
         f.ot[0] = v.x & 0xFFFFu; f.ot[1 % R] = v.x >> 16; f.ot[2 % R] = v.y & 0xFFFFu; f.ot[3 % R] = v.y >> 16;
     } else {
 #pragma unroll
-        for (int j = 0; j < R; ++j) {
-            if (phi) f.op[j] = hphi[gcell0 + j];
-            f.ot[j] = hT[gcell0 + j];
-        }
+        for (int j = 0; j < R; ++j) { f.op[j] = hphi[gcell0 + j]; f.ot[j] = hT[gcell0 + j]; }
     }
     SfHalo<TS> hm;
     sf_halo_of<TS>(hm, d, tile_x, tile_y, tc);
-    sf_halo_load<TS, true>(f.hp, f.ht, a, d, hm, hmask, phi);
+    sf_halo_load<TS, true>(f.hp, f.ht, a, d, hm, hmask);
     f.offp = a.in.off[w];
     f.offT = a.in.off[a.total_tiles + w];
     f.sclp = a.in.scl[w];
@@ -746,12 +763,11 @@ __device__ inline void sf_tile_zero(const StepArgs& a, const InstDesc& d, int gt
     }
 }
 
-// T' of a cell of a level-2 tile (sf_tile_low, sf_flat_kernel): the update
-// of sf_phase2 with phi' = +0 (one expression for both kernels, so both
-// compile it alike).
+// T' of a cell of a level-2 tile (sf_flat_kernel): the update of sf_phase2
+// with phi' = +0.
 __device__ inline float sf_t_flat(float t, float lt, float dt, float kap, float ph) {
     const float phn = 0.0f;
-    return t + dt * lt + kap * (phn - ph);
+    return sf_t_update<float>(t, lt, dt, kap, phn - ph);
 }
 
 // Phase 1 of a low tile whose phi box decodes to exact zeros (the tile and
@@ -793,15 +809,14 @@ __device__ inline void sf_phase1_flat(float* P, float* Q, float* E2, const InstD
 // all come after its own barrier A, which every thread reaches only after
 // its last read of this tile's values.
 // NOSKIP: no bulk hold and no noise skip (every term of the update is
-// evaluated, as the reference's type-1 branch does).
+// evaluated, as the reference's type-1 branch does); flat: the tile's
+// low_flat_level (0 or 1: level 1 takes sf_phase1_flat).
 template <int TS, bool NOSKIP, bool H2 = false>
 __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const StepArgs& a, const InstDesc& d,
                             int gtile, const LowFetch<TS>& f, unsigned hmask, long long step,
                             const SfTileConst<TS>* tcon = nullptr, int flat = 0) {
     using G = SfGeo<TS>;
     constexpr int H = G::H, LW = G::LW, R = G::R;
-    // phi words fetched (low_flat_level 2 and 3 tiles come without them)
-    const bool pw = !NOSKIP || flat < 2;
     const int lt = gtile - d.tile_base;
     const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
     const int ox = tile_x * TS, oy = tile_y * TS;
@@ -812,7 +827,7 @@ __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const 
     float pc[R], tc[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-        pc[j] = pw ? __half2float(__ushort_as_half(f.op[j])) * sclp : 0.0f;
+        pc[j] = __half2float(__ushort_as_half(f.op[j])) * sclp;
         tc[j] = __half2float(__ushort_as_half(f.ot[j])) * sclT;
     }
     SfHalo<TS> hm;
@@ -821,7 +836,7 @@ __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const 
     bool near0 = true, near1 = true;
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-        if (pw) L.phi[(ly + H) * LW + x0 + H + j] = pc[j];
+        L.phi[(ly + H) * LW + x0 + H + j] = pc[j];
         L.T[(ly + H) * LW + x0 + H + j] = tc[j];
         if (!NOSKIP) {
             near0 = near0 && fabsf(c0 + pc[j]) <= SF_BULK_EPS;
@@ -837,16 +852,13 @@ __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const 
             vp = (float)(raw_f64(f.hp[k].x, f.hp[k].y) - offp);
             L.T[hm.box[k]] = (float)(raw_f64(f.ht[k].x, f.ht[k].y) - offT);
         } else {
+            const unsigned short wp = (unsigned short)raw_word4(f.hp[k], hm.off[k]);
             const unsigned short wt = (unsigned short)raw_word4(f.ht[k], hm.off[k]);
-            vp = 0.0f;
-            if (pw) {
-                const unsigned short wp = (unsigned short)raw_word4(f.hp[k], hm.off[k]);
-                vp = (float)(L.nbo[slot][0][r] - offp) + __half2float(__ushort_as_half(wp)) * L.nbs[slot][0][r];
-            }
+            vp = sf_decode_rel(L.nbo[slot][0][r] - offp, wp, L.nbs[slot][0][r]);
             L.T[hm.box[k]] =
-                (float)(L.nbo[slot][1][r] - offT) + __half2float(__ushort_as_half(wt)) * L.nbs[slot][1][r];
+                sf_decode_rel(L.nbo[slot][1][r] - offT, wt, L.nbs[slot][1][r]);
         }
-        if (pw) L.phi[hm.box[k]] = vp;
+        L.phi[hm.box[k]] = vp;
         if (!NOSKIP) {
             near0 = near0 && fabsf(c0 + vp) <= SF_BULK_EPS;
             near1 = near1 && fabsf(c1 - vp) <= SF_BULK_EPS;
@@ -895,45 +907,6 @@ __device__ void sf_tile_low(SfLowLds<TS>& L, int slot, const NbCode& nxt, const 
         return;
     }
 
-    // A flat box at phi exactly 0 or 1 (low_flat_level 2; pa(1 - pa) = +-0
-    // for every cell): the flux is +0 (differences of equal P, Q and a
-    // Laplacian of zeros), pa(1 - pa)(pa - 1/2 + m) and the noise term (not
-    // taken: |pp| = 0) are +-0, so br = +0 and phi' = ph + dtt br = +0; T'
-    // takes the update of the loop with phi' - ph = +0 - ph = +0 for either
-    // sign of ph (which is why the phi words need not be fetched: ph is
-    // taken as +0).  Same values bitwise, without the ring, the flux and
-    // the atan.
-    if (NOSKIP && flat >= 2) {
-        float rp[R], rt[R], lt[R];
-        sf_lap<TS, float>(L.T, ly, x0, d.stencil == AMPS_STENCIL_REF, d.kf[SFK_INV_DXDY], tc, lt);
-        const float dt = d.kf[SFK_DT], kap = d.kf[SFK_KAPPA];
-        float mnt = INFINITY, mxt = -INFINITY;
-#pragma unroll
-        for (int i = 0; i < R; ++i) {
-            rp[i] = 0.0f;
-            rt[i] = sf_t_flat(tc[i], lt[i], dt, kap, pc[i]);
-            mnt = fminf(mnt, rt[i]);
-            mxt = fmaxf(mxt, rt[i]);
-        }
-        float mnp = 0.0f, mxp = 0.0f;
-        block_minmax2(mnp, mxp, mnt, mxt, L.red);  // C
-        const TileCode cp = tile_code(mnp, mxp);
-        const TileCode ct = tile_code(mnt, mxt);
-        const double offpn = offp + (double)cp.mid, offTn = offT + (double)ct.mid;
-        unsigned short w[R];
-        sf_encode<R>(rp, cp, w);
-        sf_store_low<R>(dphi, gcell0, w);
-        sf_encode<R>(rt, ct, w);
-        sf_store_low<R>(dT, gcell0, w);
-        if (threadIdx.x == 0) {
-            a.out.off[gtile] = offpn;
-            a.out.scl[gtile] = cp.scl;
-            a.out.off[a.total_tiles + gtile] = offTn;
-            a.out.scl[a.total_tiles + gtile] = ct.scl;
-            a.same_flags[gtile] = 0;
-        }
-        return;
-    }
     // a phi box of exact zeros needs no ring gradients (sf_phase1_flat)
     if (NOSKIP && flat == 1)
         sf_phase1_flat<TS, H2>(L.P, L.Q, L.E2, d, d.kf[SFK_COS_N_THETA0], d.kf[SFK_SIN_N_THETA0], d.kf[SFK_EPSB],
@@ -1422,7 +1395,8 @@ __global__ __launch_bounds__(256, 3) void sf_mixed_kernel(StepArgs a, const Inst
     asm volatile("" ::"s"(wfirst));  // the entry's load stays in the counts' round (not sunk into the loop)
     if (blockIdx.x == 0 && threadIdx.x == 0) {  // tile counts of the step (one writer per launch)
         a.stats[2] += (unsigned long long)nh;
-        if (!NOSKIP) a.stats[3] += (unsigned long long)nf;  // NOSKIP: the flat sweep counts every low tile
+        // NOSKIP: the flat sweep counts every low tile; these are the ones it listed
+        a.stats[NOSKIP ? 4 : 3] += (unsigned long long)nf;
     }
     for (int i = blockIdx.x; i < n; i += G) {
         if (i < nh) {
@@ -1460,70 +1434,32 @@ __global__ __launch_bounds__(256, 3) void sf_mixed_kernel(StepArgs a, const Inst
     }
 }
 
-// Full low sweep of the NO_SKIP step: one workgroup per tile of the low
-// work-list (sf_tile_low<TS, true>), in tile order.  The grid is the list's
-// upper bound, the tile count, and workgroups past the list's length exit
-// at once.  Workgroup b takes entry b: the workgroups, which the hardware
-// deals round-robin over the 8 XCDs, spread the costly tiles near the
-// crystal evenly over them (contiguous runs per XCD left the XCDs holding
-// the crystal's rows working 1.5x longer than those of the far field).
-// With no loop over tiles no per-tile constant stays live into the next
-// tile (a persistent loop with a software prefetch of the next tile kept
-// them in SGPRs spilled to VGPR lanes, needed 92 VGPRs against 48 here and
-// ran 1.2x slower), and the 6 resident workgroups of a CU (LDS-bound) hide
-// each other's load rounds.  The neighbour types and codes load first
-// (L2-resident arrays; the codes say how flat the neighbourhood is,
-// low_flat_level); a level-3 tile (sf_tile_zero) then loads nothing more,
-// a level-2 tile only its T words, other tiles phi and T.  (Loading the T
-// words with the codes made every level-3 workgroup wait for HBM loads it
-// does not use.)
-template <int TS, bool MULTI, bool H2>
-__global__ __launch_bounds__(256) void sf_low_tile_kernel(StepArgs a, const InstDesc* __restrict__ inst,
-                                                          const int* __restrict__ tinst, const int* __restrict__ wl,
-                                                          const int* __restrict__ wlc) {
-    __shared__ __align__(16) SfLowLds<TS> L;
-    const int b = blockIdx.x;
-    // entry b loads beside the count: the grid is the list's capacity, so the
-    // load stays in bounds past the count
-    const int w = wl[b];
-    const int n = *wlc;
-    if (b == 0 && threadIdx.x == 0) a.stats[3] += (unsigned long long)n;  // one writer
-    if (b >= n) return;
-    const long long step = *a.step_ctr + a.step_add;
-    SfTileConst<TS> tconst;
-    if (!MULTI) sf_tile_const<TS>(tconst, inst[0]);
-    const SfTileConst<TS>* tc = MULTI ? nullptr : &tconst;
-    const InstDesc& d = inst[MULTI ? tinst[w] : 0];
-    const int tt = nb_type_lane<TS>(a, d, w, tc);
-    NbCode c;
-    nb_code_fetch<TS>(c, a, d, w, tc);
-    const unsigned hm = hmask_of(tt);
-    const int flat = low_flat_level(c, hm);
-    if (flat == 3) {
-        sf_tile_zero<TS>(a, d, w, __shfl(c.o0, 4), __shfl(c.o1, 4));
-        return;
-    }
-    LowFetch<TS> f;
-    low_fetch<TS>(f, a, d, w, hm, tc, flat < 2);
-    nb_code_store<TS>(L, 0, c);
-    __syncthreads();  // neighbour table visible
-    // (the tile stores `c` again into slot 1, which nothing reads)
-    sf_tile_low<TS, true, H2>(L, 0, c, a, d, w, f, hm, step, tc, flat);
-}
-
 // ------------------------------------------------------------- flat sweep
 // First sweep of the NO_SKIP adaptive step: one wave per tile of the low
-// work-list (workgroup b takes entry b, tile order, as sf_low_tile_kernel
-// does).  Tiles of low_flat_level 3 and 2 are finished here, with the
-// values of sf_tile_zero and of sf_tile_low's level-2 path bitwise; a tile
-// of level 0 or 1 is appended to this step parity's list (full_seg, count
-// full_cnt[8 parity]) for the mixed sweep, which updates it beside the
-// FP64 tiles.  A level-2 tile needs only its T box in LDS (5 KB at tile
-// 32): about 30 of these workgroups fit a CU against 6 of the full low
-// tile (25 KB), so that many more tiles' load rounds overlap.  The wave
-// covers a tile as the WG threads of sf_tile_low would, in WG/64 passes
-// (virtual thread v = lane + 64 k); neighbour codes come from lanes 0..8 by
-// cross-lane reads, and the tile's min/max is a wave reduction.
+// work-list.  Workgroup b takes entry b; the list places runs of XCD_CHUNK
+// consecutive tiles on one XCD (a tile's left and right halo lines are then
+// mostly in its own XCD's L2) and deals the runs round-robin, so the costly
+// tiles near the crystal spread evenly over the XCDs (one contiguous run
+// per XCD left the XCDs holding the crystal's rows working 1.5x longer).
+//
+// Tiles of low_flat_level 3 and 2 are finished here, with the values the
+// full low update (sf_tile_low) gives them.  On a level-2 tile (the phi box
+// decodes to exact zeros at phi 0 or 1) pa(1 - pa) is +-0 in every cell;
+// the flux is +0 (differences of equal P and Q, a Laplacian of zeros),
+// pa(1 - pa)(pa - 1/2 + m) and the noise term (not taken: |pp| = 0) are
+// +-0, so br = +0 and phi' = ph + dtt br = +0, and T' is the update of
+// sf_phase2 with phi' - ph = +0 - ph = +0 for either sign of ph (so the
+// phi words need not be read: sf_t_flat).  A level-3 tile adds a T box of
+// exact zeros: sf_tile_zero.  A tile of level 0 or 1 is appended to this
+// step parity's list (full_seg, count full_cnt[8 parity]) for the mixed
+// sweep, which updates it beside the FP64 tiles.
+//
+// A level-2 tile needs only its T box in LDS (5 KB at tile 32): about 30
+// of these workgroups fit a CU against 6 of the full low tile (25 KB), so
+// that many more tiles' load rounds overlap.  The wave covers a tile as
+// the WG threads of sf_tile_low would, in WG/64 passes (virtual thread
+// v = lane + 64 k); neighbour codes come from lanes 0..8 by cross-lane
+// reads, and the tile's min/max is a wave reduction.
 constexpr int FLAT_NT = 64;
 
 template <int TS, bool MULTI>
@@ -1598,7 +1534,7 @@ __global__ __launch_bounds__(FLAT_NT) void sf_flat_kernel(StepArgs a, const Inst
         const float s1 = __shfl(c.s1, r);
         if (hmap.off[k] < 0) continue;
         const unsigned short wt = (unsigned short)raw_word4(ht[k], hmap.off[k]);
-        Tb[hmap.box[k]] = (float)(o1 - offT) + __half2float(__ushort_as_half(wt)) * s1;
+        Tb[hmap.box[k]] = sf_decode_rel(o1 - offT, wt, s1);
     }
     __syncthreads();  // the T box is in LDS
     const float dt = d.kf[SFK_DT], kap = d.kf[SFK_KAPPA];
