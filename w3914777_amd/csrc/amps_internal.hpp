@@ -115,8 +115,10 @@ struct StepArgs {
     const int* wl_low_count;
     int* same_flags;            // [total_tiles] snowflake low tiles: bit 0 both buffers hold the same phi,
                                 // bit 1 the same T (words, offset and scale)
-    int* full_seg;              // snowflake: [8][full_cap] low tiles outside the bulk-hold state this step
-    int* full_cnt;              // [2][8] entries per segment, by step parity
+    int* full_seg;              // snowflake low tiles listed for the mixed sweep this step: [8][full_cap]
+                                // (bulk-hold sweep) or [NSK_SEGS][nsk_cap] (NO_SKIP flat sweep)
+    int* full_cnt;              // entries per segment by step parity: [2][8], or NO_SKIP
+                                // [2][NSK_SEGS] at a stride of NSK_CNT_STRIDE ints
     int full_cap;
     unsigned long long* stats;  // [2] FP64 tiles, [3] low tiles given the full update, [4] NO_SKIP: low
                                 // tiles the flat sweep listed (summed over steps)
@@ -136,6 +138,13 @@ struct StepArgs {
     double* aux_in;             // model auxiliary per-cell array (grain growth: sum of squares)
     double* aux_out;
 };
+
+// NO_SKIP list of the snowflake's flat sweep: NSK_SEGS segments (list
+// position mod NSK_SEGS) of nsk_cap(total_tiles) entries, their counters
+// NSK_CNT_STRIDE ints (256 B) apart so that the appends of one launch spread
+// over as many cache lines (one line's atomics serialise).
+constexpr int NSK_SEGS = 64, NSK_CNT_STRIDE = 64;
+__host__ __device__ inline int nsk_cap(int total_tiles) { return (total_tiles + NSK_SEGS - 1) / NSK_SEGS; }
 
 // Records the message returned by amps_last_error() for the calling thread.
 void set_error_text(const char* msg);

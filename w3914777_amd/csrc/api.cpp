@@ -56,6 +56,10 @@ namespace amps {
 void set_error_text(const char* msg) { g_last_error = msg ? msg : ""; }
 }  // namespace amps
 
+// ints of the list counters: the larger of the two layouts (bulk-hold [2][8],
+// NO_SKIP [2][NSK_SEGS] at a stride of NSK_CNT_STRIDE)
+constexpr size_t FULL_CNT_INTS = 2 * (size_t)NSK_SEGS * NSK_CNT_STRIDE;
+
 struct amps_ctx {
     int model = 0, mode = 0, monitor = 0, tile = 16, every = 10, mfield = 0;
     int nfields = 0, ngroups = 1, q = 0, ninst = 0;
@@ -89,8 +93,8 @@ struct amps_ctx {
     int* d_same_flags = nullptr;           // snowflake [total_tiles], grain growth [q][total_tiles]:
                                            // tile data identical in both buffers
     size_t same_flags_n = 0;
-    int* d_full_seg = nullptr;             // [8][full_cap] snowflake: low tiles outside the bulk-hold state
-    int* d_full_cnt = nullptr;             // [2][8]
+    int* d_full_seg = nullptr;             // snowflake: low tiles listed for the mixed sweep (StepArgs::full_seg)
+    int* d_full_cnt = nullptr;             // their counts (StepArgs::full_cnt)
     int full_cap = 0;
 
     hipStream_t stream = nullptr;
@@ -632,8 +636,8 @@ int amps_create_batch(const amps_config* cfgs, int n, amps_ctx** out) {
     c->same_flags_n = (size_t)(c->model == AMPS_GRAINGROWTH ? c->q : 1) * tiles;
     A((void**)&c->d_same_flags, c->same_flags_n * sizeof(int));
     c->full_cap = (tiles + 7) / 8;
-    A((void**)&c->d_full_seg, 8 * (size_t)c->full_cap * sizeof(int));
-    A((void**)&c->d_full_cnt, 16 * sizeof(int));
+    A((void**)&c->d_full_seg, std::max(8 * (size_t)c->full_cap, (size_t)NSK_SEGS * nsk_cap(tiles)) * sizeof(int));
+    A((void**)&c->d_full_cnt, FULL_CNT_INTS * sizeof(int));
     if (rc != AMPS_OK) { std::string m = g_last_error; amps_destroy(c); return fail(rc, m); }
     e = hipMemcpy(c->d_desc, c->desc.data(), sizeof(InstDesc) * n, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(c->d_tile_inst, c->tile_inst.data(), sizeof(int) * tiles, hipMemcpyHostToDevice);
@@ -683,7 +687,8 @@ int amps_init(amps_ctx* c) {
     HIPCHK(hipMemcpyAsync(c->d_step, &zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
     static const unsigned long long z8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     HIPCHK(hipMemcpyAsync(c->d_stats, z8, sizeof(z8), hipMemcpyHostToDevice, c->stream));
-    if (c->d_full_cnt) HIPCHK(hipMemsetAsync(c->d_full_cnt, 0, 16 * sizeof(int), c->stream));  // step 0 appends to parity 0
+    // step 0 appends to parity 0
+    if (c->d_full_cnt) HIPCHK(hipMemsetAsync(c->d_full_cnt, 0, FULL_CNT_INTS * sizeof(int), c->stream));
     HIPCHK(hipMemsetAsync(c->d_same_flags, 0, c->same_flags_n * sizeof(int), c->stream));
     RCCHK(clear_zmasks(c));
     if (c->mode == AMPS_PURE_HALF2)

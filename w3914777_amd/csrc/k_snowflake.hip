@@ -1358,9 +1358,8 @@ union SfMixLds {
 
 //
 // NOSKIP: the low items are the tiles the flat sweep (sf_flat_kernel)
-// appended to this step parity's list (full_seg; counts full_cnt[0] and
-// full_cnt[8]), levels 0 and 1 of low_flat_level, each updated by the full
-// low path (H2: its half2 form).
+// appended to this step parity's NSK_SEGS segments, levels 0 and 1 of
+// low_flat_level, each updated by the full low path (H2: its half2 form).
 template <int TS, bool MULTI, bool NOSKIP, bool H2 = false>
 __global__ __launch_bounds__(256, 3) void sf_mixed_kernel(StepArgs a, const InstDesc* __restrict__ inst,
                                                       const int* __restrict__ tinst, const int* __restrict__ wl,
@@ -1374,9 +1373,11 @@ __global__ __launch_bounds__(256, 3) void sf_mixed_kernel(StepArgs a, const Inst
     const int G = gridDim.x, T = a.total_tiles;
     const int wfirst = wl[blockIdx.x < (unsigned)T ? (int)blockIdx.x : T - 1];
     int c0[8], c1[8];
+    const int lane = threadIdx.x & 63;
     if (NOSKIP) {
-        c0[0] = fcnt2[0];
-        c1[0] = fcnt2[8];
+        // lane k: segment k's count of either parity
+        c0[0] = fcnt2[lane * NSK_CNT_STRIDE];
+        c1[0] = fcnt2[(NSK_SEGS + lane) * NSK_CNT_STRIDE];
     } else {
 #pragma unroll
         for (int k = 0; k < 8; ++k) { c0[k] = fcnt2[k]; c1[k] = fcnt2[8 + k]; }
@@ -1384,9 +1385,18 @@ __global__ __launch_bounds__(256, 3) void sf_mixed_kernel(StepArgs a, const Inst
     const int nh = *wlc;
     const long long step = *a.step_ctr + a.step_add;
     const bool odd = (step & 1) != 0;
-    int cnt[8], nf = 0;
+    int cnt[8], nf = 0, sincl = 0;
     if (NOSKIP) {
-        nf = odd ? c1[0] : c0[0];
+        // inclusive prefix of the segment counts over the lanes: item j of
+        // the concatenated segments lies in the first segment whose prefix
+        // exceeds j
+        sincl = odd ? c1[0] : c0[0];
+#pragma unroll
+        for (int dlt = 1; dlt < 64; dlt <<= 1) {
+            const int t = __shfl_up(sincl, dlt);
+            if (lane >= dlt) sincl += t;
+        }
+        nf = __shfl(sincl, 63);
     } else {
 #pragma unroll
         for (int k = 0; k < 8; ++k) { cnt[k] = odd ? c1[k] : c0[k]; nf += cnt[k]; }
@@ -1409,7 +1419,10 @@ __global__ __launch_bounds__(256, 3) void sf_mixed_kernel(StepArgs a, const Inst
         } else {
             int w;
             if (NOSKIP) {
-                w = fseg[i - nh];
+                const int j = i - nh;
+                const int seg = __ffsll((long long)__ballot(sincl > j)) - 1;
+                const int excl = __shfl(sincl, seg) - __shfl(odd ? c1[0] : c0[0], seg);
+                w = fseg[seg * nsk_cap(a.total_tiles) + (j - excl)];
             } else {
                 // entry j of the concatenated segments
                 int j = i - nh, seg = 0;
@@ -1451,8 +1464,8 @@ __global__ __launch_bounds__(256, 3) void sf_mixed_kernel(StepArgs a, const Inst
 // sf_phase2 with phi' - ph = +0 - ph = +0 for either sign of ph (so the
 // phi words need not be read: sf_t_flat).  A level-3 tile adds a T box of
 // exact zeros: sf_tile_zero.  A tile of level 0 or 1 is appended to this
-// step parity's list (full_seg, count full_cnt[8 parity]) for the mixed
-// sweep, which updates it beside the FP64 tiles.
+// step parity's list (NSK_SEGS segments) for the mixed sweep, which updates
+// it beside the FP64 tiles.
 //
 // A level-2 tile needs only its T box in LDS (5 KB at tile 32): about 30
 // of these workgroups fit a CU against 6 of the full low tile (25 KB), so
@@ -1474,10 +1487,12 @@ __global__ __launch_bounds__(FLAT_NT) void sf_flat_kernel(StepArgs a, const Inst
     const int w = wl[b];
     const int n = *wlc;
     const long long step = *a.step_ctr + a.step_add;
-    if (b == 0 && threadIdx.x == 0) {
-        a.stats[3] += (unsigned long long)n;  // low tiles of the step (one writer)
-        // the next step's list: its last reader, the previous step's mixed sweep, has finished
-        a.full_cnt[(int)((step + 1) & 1) * 8] = 0;
+    if (b == 0) {
+        if (threadIdx.x == 0) a.stats[3] += (unsigned long long)n;  // low tiles of the step (one writer)
+        // the next step's counters (lane k: segment k): their last reader, the
+        // previous step's mixed sweep, has finished
+        static_assert(FLAT_NT == NSK_SEGS, "one lane per segment");
+        a.full_cnt[((int)((step + 1) & 1) * NSK_SEGS + (int)threadIdx.x) * NSK_CNT_STRIDE] = 0;
     }
     if (b >= n) return;
     const InstDesc& d = inst[MULTI ? tinst[w] : 0];
@@ -1492,7 +1507,13 @@ __global__ __launch_bounds__(FLAT_NT) void sf_flat_kernel(StepArgs a, const Inst
         return;
     }
     if (flat < 2) {
-        if (threadIdx.x == 0) a.full_seg[atomicAdd(a.full_cnt + (int)(step & 1) * 8, 1)] = w;
+        // segment = list position mod NSK_SEGS: at most nsk_cap entries each,
+        // and the appends spread over NSK_SEGS counters on their own lines
+        if (threadIdx.x == 0) {
+            const int seg = b & (NSK_SEGS - 1);
+            const int j = atomicAdd(a.full_cnt + ((int)(step & 1) * NSK_SEGS + seg) * NSK_CNT_STRIDE, 1);
+            a.full_seg[seg * nsk_cap(a.total_tiles) + j] = w;
+        }
         return;
     }
     // level 2: the T words of the tile and of its halo (no FP64 neighbour:
