@@ -11,7 +11,10 @@ Steps are chosen between monitor passes, so the tile types do not change.
 The reference branch this stands in for is snowflake/function.h:417-537
 (the type-1 half2 update), whose own arithmetic cannot be run here; the
 HALF2 cases run the engine's packed-half2 anisotropy and atan against the
-emulator's fp16 restatement of them.
+emulator's fp16 restatement of them.  In the adaptive cases the tiles whose
+3x3 neighbourhood decodes to exact zeros at phi 0 or 1 (the flat sweep's
+levels 2 and 3) must moreover come out with exactly zero phi words, scale
+0 and the same offset (and T likewise at level 3).
 
 Sintering's low tiles (the etas in fp16; c and every update in FP64) are
 checked the same way against the FP64 oracle step of the decoded state and
@@ -25,13 +28,18 @@ from tests.helpers import rel_err
 pytestmark = pytest.mark.gpu
 
 CASES = [
-    # mode, n, tile, stencil, bc, dilate, monitor threshold, HALF2
-    ("adaptive", 256, 32, 1, 1, 1, 0.05, False),
-    ("adaptive", 256, 16, 0, 1, 0, 0.05, False),
-    ("low", 128, 16, 0, 0, 0, 0.0, False),
-    ("low", 128, 32, 1, 0, 0, 0.0, False),
-    ("adaptive", 256, 32, 1, 1, 1, 0.05, True),
-    ("low", 128, 32, 0, 0, 0, 0.0, True),
+    # mode, n, tile, stencil, bc, dilate, monitor threshold, HALF2, first step
+    ("adaptive", 256, 32, 1, 1, 1, 0.05, False, 255),
+    ("adaptive", 256, 16, 0, 1, 0, 0.05, False, 255),
+    ("low", 128, 16, 0, 0, 0, 0.0, False, 255),
+    ("low", 128, 32, 1, 0, 0, 0.0, False, 255),
+    ("adaptive", 256, 32, 1, 1, 1, 0.05, True, 255),
+    ("low", 128, 32, 0, 0, 0, 0.0, True, 255),
+    # early, while phi and T are still exactly 0 away from the nucleus (by
+    # step 255 the discrete diffusion has carried subnormal values across
+    # the 256^2 grid): tiles of the flat sweep's levels 2 and 3
+    ("adaptive", 256, 32, 1, 1, 1, 0.05, False, 21),
+    ("adaptive", 256, 16, 0, 0, 0, 0.05, False, 21),
 ]
 
 
@@ -41,8 +49,37 @@ def raw_state(ctx):
     return np.stack([w0, w1]), np.stack([o0, o1]), np.stack([s0, s1])
 
 
-@pytest.mark.parametrize("mode,n,tile,stencil,bc,dil,thr,h2", CASES)
-def test_low_tiles_match_cpu_emulation(amps, O, mode, n, tile, stencil, bc, dil, thr, h2):
+def flat_levels(types, off, scl, periodic):
+    """How flat each low tile's 3x3 neighbourhood is, from the codes alone:
+    3 = phi and T decode to exact zeros there (one offset each, scale 0) with
+    the phi offset 0 or 1 as a float, 2 = phi alone, 0 otherwise (a zero-flux
+    edge maps the outside neighbour onto the tile itself, a periodic one
+    wraps)."""
+    ty, tx = types.shape
+    lev = np.zeros((ty, tx), np.int32)
+    for y in range(ty):
+        for x in range(tx):
+            nb = []
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    yy, xx = y + dy, x + dx
+                    if periodic:
+                        yy, xx = yy % ty, xx % tx
+                    else:
+                        yy, xx = min(max(yy, 0), ty - 1), min(max(xx, 0), tx - 1)
+                    nb.append((yy, xx))
+            if any(types[q] != 1 for q in nb):
+                continue
+            if not all(scl[0][q] == 0 and off[0][q] == off[0][y, x] for q in nb):
+                continue
+            if np.float32(off[0][y, x]) not in (0.0, 1.0):
+                continue
+            lev[y, x] = 3 if all(scl[1][q] == 0 and off[1][q] == off[1][y, x] for q in nb) else 2
+    return lev
+
+
+@pytest.mark.parametrize("mode,n,tile,stencil,bc,dil,thr,h2,step0", CASES)
+def test_low_tiles_match_cpu_emulation(amps, O, mode, n, tile, stencil, bc, dil, thr, h2, step0):
     c = amps.Config.default(amps.SNOWFLAKE)
     c.nx = c.ny = n
     c.tile, c.stencil, c.bc = tile, stencil, bc
@@ -56,12 +93,11 @@ def test_low_tiles_match_cpu_emulation(amps, O, mode, n, tile, stencil, bc, dil,
         c.mode, c.monitor, c.threshold = amps.ADAPTIVE, amps.MON_GRADIENT, thr
     else:
         c.mode = amps.PURE_HALF2
-    step0 = 255
     with amps.Context(c) as ctx:
         ctx.init()
         ctx.step(step0)
         ctx.sync()
-        n_low = 0
+        n_low, n_flat = 0, 0
         for k in range(3):
             types = ctx.get_tile_types()
             phi, T = ctx.get_field(0), ctx.get_field(1)
@@ -70,6 +106,18 @@ def test_low_tiles_match_cpu_emulation(amps, O, mode, n, tile, stencil, bc, dil,
             phi1, T1 = ctx.get_field(0), ctx.get_field(1)
             w1, off1, scl1 = raw_state(ctx)
             assert np.array_equal(ctx.get_tile_types(), types)  # no monitor pass in between
+            if mode == "adaptive":
+                # a tile whose neighbourhood decodes to exact zeros at phi 0 or 1
+                # gets phi' = +0 in every cell (all-zero words, scale 0, the
+                # offset kept), and with T flat too, T' = +0 likewise
+                lev = flat_levels(types, off, scl, bc == amps.BC_PERIODIC)
+                for (y, x) in zip(*np.nonzero(lev >= 2)):
+                    cells = np.s_[y * tile:(y + 1) * tile, x * tile:(x + 1) * tile]
+                    fields = (0, 1) if lev[y, x] == 3 else (0,)
+                    for f in fields:
+                        assert not w1[f][cells].any(), (k, f, y, x)
+                        assert scl1[f][y, x] == 0.0 and off1[f][y, x] == off[f][y, x], (k, f, y, x)
+                n_flat += int((lev >= 2).sum())
             ew, eo, es = O.sf_low_step(types, phi, T, w, off, scl, p, tile, stencil, bc, seed=c.seed, step=step0 + k,
                                        half2=h2)
             low = np.repeat(np.repeat(types == 1, tile, 0), tile, 1)
@@ -95,6 +143,8 @@ def test_low_tiles_match_cpu_emulation(amps, O, mode, n, tile, stencil, bc, dil,
                 assert rel_err(phi1[hi], po[hi]) <= 1e-12
                 assert rel_err(T1[hi], to[hi]) <= 1e-12
         assert n_low > 0
+        if step0 < 100:
+            assert n_flat > 0  # the flat sweep's shortcuts were taken
 
 
 @pytest.mark.parametrize("bc,tile,cons", [(0, 32, False), (1, 16, False), (1, 32, True)])
