@@ -154,9 +154,14 @@ __device__ inline Rt sf_t_update(Rt t, Rt lt, Rt dt, Rt kap, Rt dphi) {
 }
 
 // A neighbour's fp16 word as a deviation from this tile's offset:
-// (float)(its offset - ours) + word * its scale.
+// (float)(its offset - ours) + word * its scale.  The conversion is kept a
+// separate instruction (the empty asm pins its result in a VGPR), so every
+// site rounds with the same v_cvt_f32_f16 + v_fma_f32 pair rather than a
+// v_fma_mix_f32 at some sites and the pair at others.
 __device__ inline float sf_decode_rel(double doff, unsigned short w, float scl) {
-    return fmaf(__half2float(__ushort_as_half(w)), scl, (float)doff);
+    float h = __half2float(__ushort_as_half(w));
+    asm volatile("" : "+v"(h));
+    return fmaf(h, scl, (float)doff);
 }
 
 template <int TS, typename Rt, int NT = WG>
