@@ -1272,60 +1272,29 @@ __global__ __launch_bounds__(64) void sf_hold_kernel(StepArgs a, const InstDesc*
 }
 
 // ---------------------------------------------------------------- kernels
-// Persistent sweeps over a tile work-list: the FP64 kernel takes every tile
-// (PURE_FP64) or the FP64 list, the low kernel the low-precision list; each
-// list is built on the device after every classification, in tile order.
+// Sweeps over device-built tile work-lists (rebuilt after every
+// classification): the FP64 list and the low-precision list.
 //
 // The read-only metadata (instance descriptors, tile -> instance map, the
 // work-list and its count) arrive as separate restrict-qualified arguments:
 // the compiler then reads them through the scalar cache instead of a chain of
 // vector loads and waits per tile, and with MULTI = false (one instance) the
-// descriptor is loaded once per workgroup.  The next work-list entry is read
-// one tile ahead.
+// descriptor is loaded once per workgroup.
 // Type of neighbour tile (lane % 64) of tile w, for lanes 0..8 of every wave
 // (2 elsewhere); hmask_of() turns it into the wave-uniform 9-bit mask of FP64
-// neighbours.  Issued one tile ahead, so the halo loads of a tile never wait
-// on a type load.
+// neighbours.
 template <int TS>
 __device__ inline int nb_type_lane(const StepArgs& a, const InstDesc& d, int w, const SfTileConst<TS>* tc = nullptr) {
     const int lane = threadIdx.x & 63;
     if (a.type == nullptr) return 2;
     // every lane loads (lanes 9.. repeat neighbour 8): no branch around the
-    // load, so the wait for it can be placed exactly one tile later
+    // load, so its wait lands where the type is used
     const int t = a.type[nb_tile_of<TS>(d, w, tc)];
     return lane < 9 ? t : 2;
 }
 
 __device__ inline unsigned hmask_of(int t) {
     return (unsigned)(__ballot((threadIdx.x & 63) < 9 && t == 2) & 0x1FFull);
-}
-
-template <int TS, bool ADAPT, bool MULTI, int NT = WG>
-__global__ __launch_bounds__(NT, 3) void sf_high_kernel(StepArgs a, const InstDesc* __restrict__ inst,
-                                                         const int* __restrict__ tinst, const int* __restrict__ wl,
-                                                         const int* __restrict__ wlc) {
-    __shared__ __align__(16) SfLds<TS, double> L;
-    const long long step = *a.step_ctr + a.step_add;
-    const int n = wlc ? *wlc : a.nwork, G = gridDim.x;
-    int b = blockIdx.x;
-    // tiles updated (adaptive steps; one writer per launch)
-    if (ADAPT && a.stats && b == 0 && threadIdx.x == 0) a.stats[2] += (unsigned long long)n;
-    if (b >= n) return;
-    // list entries are placed for XCD runs (worklist_kernel): read in order
-    int w = wl ? wl[b] : xcd_remap(b, n);
-    int tcur = ADAPT ? nb_type_lane<TS>(a, inst[MULTI ? tinst[w] : 0], w) : 2;
-    for (; b < n; b += G) {
-        const bool more = b + G < n;
-        const int wn = more ? (wl ? wl[b + G] : xcd_remap(b + G, n)) : w;
-        const int tnext = ADAPT && more ? nb_type_lane<TS>(a, inst[MULTI ? tinst[wn] : 0], wn) : 2;
-        const InstDesc& d = inst[MULTI ? tinst[w] : 0];
-        const int lt = w - d.tile_base;
-        const int tile_y = lt / d.tx, tile_x = lt - tile_y * d.tx;
-        sf_tile_f64<TS, ADAPT, NT>(L, a, d, tile_x, tile_y, ADAPT ? hmask_of(tcur) : 0x1FFu, step);
-        __syncthreads();
-        w = wn;
-        tcur = tnext;
-    }
 }
 
 // PURE_FP64 sweep: one workgroup per tile (the grid is the work count).
