@@ -101,6 +101,10 @@ enum {
  *             to exactly 0, and grain interiors are held.  The reference updates
  *             every type-1 tile every step (snowflake/function.h:376-604,
  *             graingrowth/function.h:33-277): NO_SKIP is that schedule.
+ *             (Under NO_SKIP a snowflake low tile whose 3x3 neighbourhood
+ *             decodes to exact zeros at phi 0 or 1 gets phi' = +0 and T' from
+ *             the T update alone: the values the full update gives, without
+ *             the terms that are signed zeros there.)
  * CH_CONSERVATIVE : sintering Cahn-Hilliard update in divergence form,
  *             c' = c + dt div(M grad mu) with face mobilities
  *             M_{i+1/2} = (M_i + M_{i+1})/2, in FP64 (mass-conserving up to
